@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""Benchmark: fp16 batched 1D FFT, N = 2^16, batch 256 (the headline workload).
+
+One step = one tcfftExec over the whole batch (256 signals of 65536 complex
+fp16 values, in place, digit-reversed input, natural output) on one MI355X.
+Inputs are synthetic uniform[-1,1) fp16 and already resident in HBM; the timed
+steps rotate over ROTATE buffers (> 512 MiB in total) so no step starts with
+its data in the 256 MB MALL.
+
+Prints one JSON line:
+  value      = 5*N*log2(N)*batch / step time, GFLOP/s
+  roofline   = dominant pass kernel: algorithmic bytes per launch
+               (8 B per complex element: one fp16 read + one fp16 write)
+               / its mean launch time (HIP events on the launch stream),
+               against 8 TB/s; traffic from rocprofv3 counters when a
+               profile summary is committed, else null
+  cpu_baseline = the FP64 checker (oracle/, a scalar C restatement of the
+               reference algorithm) on a bounded sample, 1 host core
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+N = 1 << 16
+BATCH = 256
+ROTATE = 10          # 10 x 64 MiB = 640 MiB
+PEAK_HBM_GBS = 8000.0
+METRIC = "fp16 FFT GFLOP/s (5N·log2N) + % HBM roofline at N=2^16; rel-L2 vs FP64"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=N)
+    ap.add_argument("--batch", type=int, default=BATCH)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    return ap.parse_args()
+
+
+def cpu_baseline(n: int):
+    """FP64 checker on a bounded sample of the same workload (1 core)."""
+    import oracle
+    oracle.lib()
+    rng = np.random.default_rng(0)
+    sig = 4
+    x = (rng.uniform(-1, 1, (sig, n)) + 1j * rng.uniform(-1, 1, (sig, n))).astype(np.complex64).astype(np.complex128)
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        oracle.fft(x)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > 10.0 or reps >= 50:
+            break
+    flops = 5.0 * n * math.log2(n) * sig * reps
+    return {"value": round(flops / el / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} x {sig} signals of N={n} (FP64 DIT restatement, {el:.1f} s)"}
+
+
+def load_traffic(n: int, batch: int):
+    """Counter-derived HBM bytes per launch of the dominant kernel, if profiled."""
+    path = os.path.join(ROOT, "profiles", f"traffic_n{n}_b{batch}.json")
+    if not os.path.exists(path):
+        return None
+    try:
+        with open(path) as f:
+            return json.load(f).get("bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batch axis shards with "
+              f"no exchange, see DESIGN.md. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import tcfft_amd
+
+    n, batch = args.n, args.batch
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    plan = tcfft_amd.Plan1D(n, batch)
+    stream = torch.cuda.current_stream()
+    sptr = stream.cuda_stream
+
+    # synthetic fp16 inputs, digit-reversed by the plan's radices (gen_rev)
+    rng = np.random.default_rng(0)
+    x = rng.uniform(-1, 1, (batch, n, 2)).astype(np.float16)
+    rev = tcfft_amd.gen_rev(n, plan.radices)
+    x_rev = np.ascontiguousarray(x[:, rev, :])
+    host = torch.from_numpy(x_rev)
+    bufs = [torch.empty((batch, n, 2), dtype=torch.float16, device=dev) for _ in range(ROTATE)]
+    for b in bufs:
+        b.copy_(host)
+    torch.cuda.synchronize()
+
+    # accuracy of one exec vs FP64 (numpy pocketfft) on a few signals
+    check = bufs[0]
+    plan.exec(check.data_ptr(), tcfft_amd.FORWARD, sptr)
+    torch.cuda.synchronize()
+    k = min(batch, 8)
+    got = check[:k].float().cpu().numpy().astype(np.float64)
+    gotc = got[..., 0] + 1j * got[..., 1]
+    xin = x[:k].astype(np.float64)
+    ref = np.fft.fft(xin[..., 0] + 1j * xin[..., 1], axis=-1)
+    rel_l2 = float(np.linalg.norm(gotc - ref) / np.linalg.norm(ref))
+    check.copy_(host)
+
+    # whole-exec timing
+    for i in range(args.warmup):
+        plan.exec(bufs[i % ROTATE].data_ptr(), tcfft_amd.FORWARD, sptr)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    t_wall0 = time.perf_counter()
+    e0.record(stream)
+    for i in range(args.steps):
+        plan.exec(bufs[i % ROTATE].data_ptr(), tcfft_amd.FORWARD, sptr)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    t_wall = time.perf_counter() - t_wall0
+    ms = e0.elapsed_time(e1) / args.steps
+
+    # per-pass timing (each pass kernel alone, same rotation, same stream)
+    npass = plan.num_passes
+    per_pass = []
+    for p in range(npass):
+        reps = max(20, args.steps // 2)
+        for i in range(5):
+            plan.exec_passes(bufs[i % ROTATE].data_ptr(), p, 1, tcfft_amd.FORWARD, sptr)
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(stream)
+        for i in range(reps):
+            plan.exec_passes(bufs[i % ROTATE].data_ptr(), p, 1, tcfft_amd.FORWARD, sptr)
+        b.record(stream)
+        torch.cuda.synchronize()
+        per_pass.append(a.elapsed_time(b) / reps)
+    dom = int(np.argmax(per_pass))
+    alg_bytes = 8.0 * n * batch
+    achieved = alg_bytes / (per_pass[dom] * 1e-3) / 1e9
+    exec_gbs = alg_bytes / (ms * 1e-3) / 1e9
+
+    flops = 5.0 * n * math.log2(n) * batch
+    gflops = flops / (ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(gflops, 1),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f16 (FP32 MFMA accumulate)",
+        "data": "synthetic uniform[-1,1) fp16, resident in HBM, rotated over %d buffers" % ROTATE,
+        "config": {"workload": f"1D C2C FFT N={n} batch={batch}, fp16 interleaved, in place, "
+                               f"digit-reversed input (tcfftExec)", "n": n, "batch": batch,
+                   "passes": npass, "radices": plan.radices},
+        "rel_l2_vs_fp64": rel_l2,
+        "hbm_frac_whole_exec": round(exec_gbs / PEAK_HBM_GBS, 4),
+        "pass_ms": [round(v, 5) for v in per_pass],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(n, batch),
+                     "kernel": f"pass {dom}"},
+        "wall_s_timed": round(t_wall, 3),
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n)
+    plan.destroy()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

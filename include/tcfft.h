@@ -1,0 +1,98 @@
+/*
+ * tcfft.h -- C ABI of the MI355X (gfx950) half-precision FFT library
+ * (libtcfft.so).  Plain pointers, sizes and status codes; no HIP or torch
+ * types.  `stream` arguments are hipStream_t values passed as void*
+ * (NULL = the null stream, as the reference launches on).
+ *
+ * Data: interleaved FP16 complex (re, im halves), signal b of length n at
+ * element offset n*b; 2D images row-major [batch][nx][ny] with ny contiguous,
+ * exactly the buffers of tcfft_half.h / tcfft_half_2d.h.  Forward is
+ * exp(-2*pi*i*nk/N) (FFTW_FORWARD, tcfft_half.cu:40-44), unnormalised.
+ *
+ * The C++ drop-in headers tcfft_half.h and tcfft_half_2d.h are thin inline
+ * shims over these entry points.
+ */
+#ifndef TCFFT_H
+#define TCFFT_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct tcfft_plan_s *tcfft_plan;
+
+enum tcfft_status {
+    TCFFT_SUCCESS = 0,
+    TCFFT_INVALID_SIZE = 1,    /* n / nx / ny not a supported power of two */
+    TCFFT_INVALID_BATCH = 2,   /* batch < 1 or total elements >= 2^31 */
+    TCFFT_INVALID_POINTER = 3, /* NULL or misaligned (< 16 B) buffer, NULL out-parameter */
+    TCFFT_INVALID_PLAN = 4,    /* NULL plan, or a plan of the other kind */
+    TCFFT_INVALID_VALUE = 5,   /* bad direction, axis, capacity ... */
+    TCFFT_HIP_ERROR = 6,       /* a HIP runtime call failed (see tcfft_last_hip_error) */
+    TCFFT_ALLOC_FAILED = 7
+};
+
+enum tcfft_direction { TCFFT_FORWARD = -1, TCFFT_INVERSE = +1 };
+
+/* Library version, e.g. 100 for 0.1.0. */
+int tcfft_version(void);
+
+/* Human-readable name of a status code (static storage). */
+const char *tcfft_status_string(int status);
+
+/* Last HIP error code seen by the library on this thread (0 if none). */
+int tcfft_last_hip_error(void);
+
+/* Replaces tcfftCreate(tcfftHandle*, int n, int n_batch), tcfft_half.h:21 /
+ * tcfft_half.cu:1235-1345.  n is a power of two, 256 <= n <= 2^27; the
+ * reference's table covers only 10 of these sizes (tcfft_half.cu:1250-1330).
+ * Uploads FP64-generated twiddle tables to the current HIP device. */
+int tcfft_plan_create_1d(tcfft_plan *plan, int n, int n_batch);
+
+/* Replaces tcfftCreate(tcfftHandle*, int nx, int ny, int n_batch),
+ * tcfft_half_2d.h:21 / tcfft_half_2d.cu:1151-1221.  256 <= ny <= 2^16 and
+ * 2 <= nx <= 2^16 powers of two (the reference: nx in {256,512},
+ * ny in {256,512,1024}). */
+int tcfft_plan_create_2d(tcfft_plan *plan, int nx, int ny, int n_batch);
+
+/* The reference never frees a plan (tcfft_half.cu:1331-1344 leaks); this does. */
+int tcfft_plan_destroy(tcfft_plan plan);
+
+/* Digit-reversal radix list the plan's passes consume along `axis`
+ * (1D: axis 0; 2D: axis 0 = x (columns, length nx), axis 1 = y (rows, length
+ * ny)).  Replaces reading tcfftHandle::radices / n_radices
+ * (tcfft_doit_half.cpp:38) and radices_x/_y (tcfft_doit_half_2d.cpp:40-41). */
+int tcfft_plan_radices(tcfft_plan plan, int axis, int *radices, int capacity, int *n_radices);
+
+/* Host-only query of the 1D plan structure (no device access): the radix
+ * list and the number of passes tcfft_plan_create_1d would build. */
+int tcfft_query_1d(int n, int n_batch, int *radices, int capacity, int *n_radices, int *n_passes);
+int tcfft_query_2d(int nx, int ny, int n_batch, int *radices_x, int *n_radices_x, int *radices_y,
+                   int *n_radices_y, int *n_passes, int capacity);
+
+/* Replaces tcfftExec(tcfftHandle, half *data), tcfft_half.h:20 /
+ * tcfft_half.cu:1177-1233 (and the 2D one, tcfft_half_2d.cu:1128-1149):
+ * in place on a device buffer whose input is digit-reversed by the plan's
+ * radix list (per axis for 2D); output in natural order.  direction is
+ * TCFFT_FORWARD or TCFFT_INVERSE (unnormalised adjoint).  Asynchronous on
+ * `stream`. */
+int tcfft_exec(tcfft_plan plan, void *data, int direction, void *stream);
+
+/* Natural-order input and output.  in == out is allowed (uses a plan-owned
+ * device workspace of the buffer's size, allocated on first use). */
+int tcfft_exec_natural(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
+
+/* Runs passes [first, first + count) of the plan only (the buffer must hold
+ * what the earlier passes produced).  For profiling individual pass kernels;
+ * tcfft_exec == tcfft_exec_passes(plan, data, dir, 0, tcfft_plan_num_passes, s). */
+int tcfft_plan_num_passes(tcfft_plan plan, int *n_passes);
+int tcfft_exec_passes(tcfft_plan plan, void *data, int direction, int first, int count, void *stream);
+
+/* Host helper with the semantics of the caller-side gen_rev
+ * (tcfft_doit_half.cpp:6-29): rev[i] = source index of position i. */
+int tcfft_gen_rev(int n, const int *radices, int n_radices, int *rev);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,210 @@
+"""tcfft_amd -- MI355X (gfx950) half-precision FFT, Python view of the C ABI.
+
+The product is libtcfft.so (HIP kernels + C++ plan builder behind the C ABI in
+include/tcfft.h).  This module binds that ABI with ctypes and offers:
+
+  * Plan1D / Plan2D: the tcfftCreate / tcfftExec pair of tcfft_half.h and
+    tcfft_half_2d.h (same arguments: sizes, batch, an in-place device buffer
+    whose input is digit-reversed by plan.radices), plus natural-order and
+    adjoint execution;
+  * gen_rev: the caller-side digit reversal of tcfft_doit_half.cpp:6-29;
+  * torch helpers (tcfft_amd.torch_fft) that pass tensor pointers and the
+    current HIP stream.
+
+There is no CPU fallback: if libtcfft.so is missing or fails to load, every
+entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtcfft.so")
+
+FORWARD = -1
+INVERSE = +1
+
+STATUS = {
+    0: "TCFFT_SUCCESS", 1: "TCFFT_INVALID_SIZE", 2: "TCFFT_INVALID_BATCH", 3: "TCFFT_INVALID_POINTER",
+    4: "TCFFT_INVALID_PLAN", 5: "TCFFT_INVALID_VALUE", 6: "TCFFT_HIP_ERROR", 7: "TCFFT_ALLOC_FAILED",
+}
+
+# Every extern "C" symbol declared in include/tcfft.h.
+EXPORTS = (
+    "tcfft_version", "tcfft_status_string", "tcfft_last_hip_error", "tcfft_plan_create_1d",
+    "tcfft_plan_create_2d", "tcfft_plan_destroy", "tcfft_plan_radices", "tcfft_query_1d", "tcfft_query_2d",
+    "tcfft_exec", "tcfft_exec_natural", "tcfft_gen_rev", "tcfft_plan_num_passes", "tcfft_exec_passes",
+)
+
+
+class TcfftError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        name = STATUS.get(status, f"status {status}")
+        extra = ""
+        if status == 6 and _lib is not None:
+            extra = f" (hip error {_lib.tcfft_last_hip_error()})"
+        super().__init__(f"{what}: {name}{extra}")
+
+
+_lib = None
+_i32p = ctypes.POINTER(ctypes.c_int)
+_vp = ctypes.c_void_p
+
+
+def lib():
+    """Load libtcfft.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m tcfft_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.tcfft_version.restype = ctypes.c_int
+        L.tcfft_status_string.argtypes = [ctypes.c_int]
+        L.tcfft_status_string.restype = ctypes.c_char_p
+        L.tcfft_plan_create_1d.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]
+        L.tcfft_plan_create_2d.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.tcfft_plan_destroy.argtypes = [_vp]
+        L.tcfft_plan_radices.argtypes = [_vp, ctypes.c_int, _i32p, ctypes.c_int, _i32p]
+        L.tcfft_query_1d.argtypes = [ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p]
+        L.tcfft_query_2d.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, _i32p, _i32p, _i32p,
+                                     _i32p, ctypes.c_int]
+        L.tcfft_exec.argtypes = [_vp, _vp, ctypes.c_int, _vp]
+        L.tcfft_exec_natural.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+        L.tcfft_gen_rev.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, _i32p]
+        L.tcfft_plan_num_passes.argtypes = [_vp, _i32p]
+        L.tcfft_exec_passes.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise TcfftError(rc, what)
+
+
+def version() -> int:
+    return lib().tcfft_version()
+
+
+def gen_rev(n: int, radices) -> np.ndarray:
+    """Source index of every position (tcfft_doit_half.cpp:6-29 semantics)."""
+    r = np.ascontiguousarray(radices, dtype=np.int32)
+    out = np.empty(n, dtype=np.int32)
+    _check(lib().tcfft_gen_rev(n, r.ctypes.data_as(_i32p), len(r), out.ctypes.data_as(_i32p)), "tcfft_gen_rev")
+    return out
+
+
+def query_1d(n: int, batch: int = 1):
+    """Host-only: (radices, n_passes) the 1D plan would use."""
+    rad = np.zeros(32, dtype=np.int32)
+    nr, npass = ctypes.c_int(), ctypes.c_int()
+    _check(lib().tcfft_query_1d(n, batch, rad.ctypes.data_as(_i32p), 32, ctypes.byref(nr), ctypes.byref(npass)),
+           f"tcfft_query_1d(n={n}, batch={batch})")
+    return [int(v) for v in rad[: nr.value]], npass.value
+
+
+def query_2d(nx: int, ny: int, batch: int = 1):
+    rx = np.zeros(32, dtype=np.int32)
+    ry = np.zeros(32, dtype=np.int32)
+    nrx, nry, npass = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+    _check(lib().tcfft_query_2d(nx, ny, batch, rx.ctypes.data_as(_i32p), ctypes.byref(nrx),
+                                ry.ctypes.data_as(_i32p), ctypes.byref(nry), ctypes.byref(npass), 32),
+           f"tcfft_query_2d(nx={nx}, ny={ny}, batch={batch})")
+    return [int(v) for v in rx[: nrx.value]], [int(v) for v in ry[: nry.value]], npass.value
+
+
+class _Plan:
+    dims = 0
+
+    def __init__(self):
+        self._h = _vp()
+
+    @property
+    def handle(self):
+        return self._h.value
+
+    def _radices(self, axis: int):
+        rad = np.zeros(32, dtype=np.int32)
+        nr = ctypes.c_int()
+        _check(lib().tcfft_plan_radices(self._h, axis, rad.ctypes.data_as(_i32p), 32, ctypes.byref(nr)),
+               "tcfft_plan_radices")
+        return [int(v) for v in rad[: nr.value]]
+
+    def exec(self, data_ptr: int, direction: int = FORWARD, stream: int = 0):
+        """In place; input digit-reversed by the plan's radices (tcfftExec)."""
+        _check(lib().tcfft_exec(self._h, _vp(data_ptr), direction, _vp(stream)), "tcfft_exec")
+
+    @property
+    def num_passes(self) -> int:
+        n = ctypes.c_int()
+        _check(lib().tcfft_plan_num_passes(self._h, ctypes.byref(n)), "tcfft_plan_num_passes")
+        return n.value
+
+    def exec_passes(self, data_ptr: int, first: int, count: int, direction: int = FORWARD, stream: int = 0):
+        """Passes [first, first+count) only (profiling of individual pass kernels)."""
+        _check(lib().tcfft_exec_passes(self._h, _vp(data_ptr), direction, first, count, _vp(stream)),
+               "tcfft_exec_passes")
+
+    def exec_natural(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
+        _check(lib().tcfft_exec_natural(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)),
+               "tcfft_exec_natural")
+
+    def destroy(self):
+        if self._h.value:
+            _check(lib().tcfft_plan_destroy(self._h), "tcfft_plan_destroy")
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h.value and _lib is not None:
+                _lib.tcfft_plan_destroy(self._h)
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.destroy()
+
+
+class Plan1D(_Plan):
+    """tcfftCreate(&plan, n, n_batch) of tcfft_half.h."""
+
+    dims = 1
+
+    def __init__(self, n: int, batch: int = 1):
+        super().__init__()
+        self.n, self.batch = n, batch
+        _check(lib().tcfft_plan_create_1d(ctypes.byref(self._h), n, batch),
+               f"tcfft_plan_create_1d(n={n}, batch={batch})")
+        self.radices = self._radices(0)
+        self.n_radices = len(self.radices)
+
+    @property
+    def numel(self) -> int:
+        return self.n * self.batch
+
+
+class Plan2D(_Plan):
+    """tcfftCreate(&plan, nx, ny, n_batch) of tcfft_half_2d.h."""
+
+    dims = 2
+
+    def __init__(self, nx: int, ny: int, batch: int = 1):
+        super().__init__()
+        self.nx, self.ny, self.batch = nx, ny, batch
+        _check(lib().tcfft_plan_create_2d(ctypes.byref(self._h), nx, ny, batch),
+               f"tcfft_plan_create_2d(nx={nx}, ny={ny}, batch={batch})")
+        self.radices_x = self._radices(0)
+        self.radices_y = self._radices(1)
+        self.n_radices_x = len(self.radices_x)
+        self.n_radices_y = len(self.radices_y)
+
+    @property
+    def numel(self) -> int:
+        return self.nx * self.ny * self.batch

@@ -1,0 +1,41 @@
+// fft_kernels.hpp -- launch interface between the pass driver and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace tcfft {
+
+// Twiddle that a pass applies to its FP32 results for the pass that follows
+// (see epi_twiddle in fft_kernels.hip).  hi/lo are FP64-generated FP32 tables
+// of exp(-2*pi*i*k/M) split as W_M^m = hi[m >> lo_bits] * lo[m & (2^lo_bits-1)].
+struct EpiParams {
+    const float2* hi;
+    const float2* lo;
+    uint32_t lo_bits;
+    uint32_t m_mask;   // M - 1, M = s * R of the next pass
+    uint32_t shift;    // axis position = element index >> shift
+    uint32_t s_log2;   // next pass step, in axis positions
+    uint32_t r_log2;   // next pass radix
+    uint32_t ra_log2;  // first radix inside the next pass (DIT order)
+};
+
+// First pass: contiguous lines of 256*T elements (T = 1, 2, 4, 8, 16).
+hipError_t launch_rows(int T, uint32_t* buf, uint32_t n_tiles, int sign, const float2* w4096,
+                       const EpiParams* epi, hipStream_t st);
+
+// Strided pass of radix R (2..256) at element step 2^s_log2 over `outer` blocks.
+hipError_t launch_cols(int R, uint32_t* buf, uint32_t s_log2, uint32_t outer, int sign, const float2* w4096,
+                       const EpiParams* epi, hipStream_t st);
+
+// Digit-reversal gather (natural-order input).  rev digits are described by
+// the radix list (up to 12 digits per axis).
+struct RevSpec {
+    int n_digits;
+    int radix[12];
+};
+hipError_t launch_gather_rev_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
+                                const RevSpec& spec, hipStream_t st);
+hipError_t launch_gather_rev_2d(const uint32_t* in, uint32_t* out, uint32_t nx_log2, uint32_t ny_log2,
+                                uint32_t total, const RevSpec& sx, const RevSpec& sy, hipStream_t st);
+
+}  // namespace tcfft

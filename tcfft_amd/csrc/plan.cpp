@@ -1,0 +1,445 @@
+// plan.cpp -- plan builder, pass driver and C ABI (include/tcfft.h).
+//
+// The reference's tcfftCreate fills a fixed size table and a kernel pointer
+// table (tcfft_half.cu:1235-1345); tcfftExec launches one first-pass kernel
+// then strided passes with a growing step (tcfft_half.cu:1177-1233).  Here
+// the plan is computed for any power of two: a first pass over contiguous
+// lines of L = min(n, 4096) (one wave per 4096 elements, all in registers),
+// then strided passes of radix <= 256 in 8-bit chunks.  Each pass but the
+// last multiplies its FP32 results by the twiddle of the pass that follows,
+// from FP64-generated tables uploaded at plan creation.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "../../include/tcfft.h"
+#include "fft_kernels.hpp"
+
+using tcfft::EpiParams;
+
+namespace {
+
+constexpr uint32_t kMagic = 0x7cff7a11u;
+thread_local int g_last_hip_error = 0;
+
+struct Pass {
+    int kind = 0;          // 0: contiguous rows, 1: strided columns
+    int radix = 0;         // rows: T (line = 256*T); columns: R
+    uint32_t n_tiles = 0;  // rows
+    uint32_t s_log2 = 0;   // columns: element step
+    uint32_t outer = 0;    // columns: number of [R][s] blocks
+    bool has_epi = false;
+    EpiParams epi{};
+    uint32_t epi_M_log2 = 0;  // table size to bind at upload time
+};
+
+struct Layout {  // host-only result of the plan builder
+    int dims = 1;
+    int n = 0, nx = 0, ny = 0, batch = 0;
+    std::vector<int> rad_x, rad_y;  // 1D: rad_y holds the list
+    std::vector<Pass> passes;
+};
+
+int ilog2(uint32_t v) {
+    int r = 0;
+    while ((1u << r) < v) ++r;
+    return r;
+}
+bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
+
+// Split `bits` into pass radices of at most 2^8.
+std::vector<int> split_bits(int bits) {
+    std::vector<int> out;
+    while (bits > 0) {
+        int b = bits > 8 ? 8 : bits;
+        out.push_back(1 << b);
+        bits -= b;
+    }
+    return out;
+}
+
+// DIT radix digits consumed inside one strided pass of radix R.
+void pass_digits(int R, std::vector<int>& rad) {
+    if (R >= 16) {
+        rad.push_back(16);
+        if (R > 16) rad.push_back(R / 16);
+    } else {
+        rad.push_back(R);
+    }
+}
+
+// Epilogue description for a next pass (radix R at axis step s, shift).
+void set_epi(Pass& p, uint32_t s_log2_axis, int R_next, uint32_t shift) {
+    p.has_epi = true;
+    p.epi.shift = shift;
+    p.epi.s_log2 = s_log2_axis;
+    p.epi.r_log2 = (uint32_t)ilog2((uint32_t)R_next);
+    p.epi.ra_log2 = R_next >= 16 ? 4u : p.epi.r_log2;
+    p.epi_M_log2 = s_log2_axis + p.epi.r_log2;
+    p.epi.m_mask = (p.epi_M_log2 >= 32) ? 0xffffffffu : ((1u << p.epi_M_log2) - 1u);
+    p.epi.lo_bits = p.epi_M_log2 / 2;
+}
+
+// Passes for contiguous transforms of length n over `lines` lines.
+int build_axis_rows(int n, long lines, std::vector<int>& rad, std::vector<Pass>& passes) {
+    const int nb = ilog2((uint32_t)n);
+    const int L = n < 4096 ? n : 4096;
+    const int T = L / 256;
+    Pass first;
+    first.kind = 0;
+    first.radix = T;
+    first.n_tiles = (uint32_t)((long)n * lines / 256);
+    rad.push_back(16);
+    rad.push_back(16);
+    if (T > 1) rad.push_back(T);
+    passes.push_back(first);
+    uint32_t s_log2 = (uint32_t)ilog2((uint32_t)L);
+    for (int R : split_bits(nb - (int)s_log2)) {
+        set_epi(passes.back(), s_log2, R, 0);
+        Pass p;
+        p.kind = 1;
+        p.radix = R;
+        p.s_log2 = s_log2;
+        p.outer = (uint32_t)((long)n * lines / ((long)R << s_log2));
+        pass_digits(R, rad);
+        passes.push_back(p);
+        s_log2 += (uint32_t)ilog2((uint32_t)R);
+    }
+    return 0;
+}
+
+int build_layout_1d(int n, int batch, Layout& out) {
+    if (!is_pow2(n) || n < 256 || n > (1 << 27)) return TCFFT_INVALID_SIZE;
+    if (batch < 1 || (long)n * batch >= (1L << 31)) return TCFFT_INVALID_BATCH;
+    out = Layout{};
+    out.dims = 1;
+    out.n = n;
+    out.batch = batch;
+    build_axis_rows(n, batch, out.rad_y, out.passes);
+    return TCFFT_SUCCESS;
+}
+
+int build_layout_2d(int nx, int ny, int batch, Layout& out) {
+    if (!is_pow2(nx) || !is_pow2(ny) || ny < 256 || ny > (1 << 16) || nx < 2 || nx > (1 << 16))
+        return TCFFT_INVALID_SIZE;
+    if (batch < 1 || (long)nx * ny * batch >= (1L << 31)) return TCFFT_INVALID_BATCH;
+    out = Layout{};
+    out.dims = 2;
+    out.nx = nx;
+    out.ny = ny;
+    out.batch = batch;
+    // rows: length ny, nx*batch lines; no twiddle between the two axes
+    build_axis_rows(ny, (long)nx * batch, out.rad_y, out.passes);
+    // columns: strided passes along x with memory step ny * s_x
+    const uint32_t ny_log2 = (uint32_t)ilog2((uint32_t)ny);
+    uint32_t sx_log2 = 0;
+    bool first = true;
+    for (int R : split_bits(ilog2((uint32_t)nx))) {
+        if (!first) set_epi(out.passes.back(), sx_log2, R, ny_log2);
+        first = false;
+        Pass p;
+        p.kind = 1;
+        p.radix = R;
+        p.s_log2 = ny_log2 + sx_log2;
+        p.outer = (uint32_t)((long)nx * batch / ((long)R << sx_log2));
+        pass_digits(R, out.rad_x);
+        out.passes.push_back(p);
+        sx_log2 += (uint32_t)ilog2((uint32_t)R);
+    }
+    return TCFFT_SUCCESS;
+}
+
+}  // namespace
+
+struct tcfft_plan_s {
+    uint32_t magic = kMagic;
+    Layout lay;
+    int device = 0;
+    float2* w4096 = nullptr;
+    std::map<uint32_t, std::pair<float2*, float2*>> epi_tables;  // M_log2 -> (hi, lo)
+    std::vector<void*> allocs;
+    void* workspace = nullptr;
+    size_t workspace_bytes = 0;
+    size_t total = 0;  // elements in the buffer
+};
+
+namespace {
+
+int hip_fail(hipError_t e) {
+    g_last_hip_error = (int)e;
+    return TCFFT_HIP_ERROR;
+}
+
+// exp(-2*pi*i*k/M) for k in [k0, k0 + count) with stride `step`, computed in
+// FP64 and rounded to FP32.
+std::vector<float2> host_table(uint32_t M_log2, uint64_t step, uint32_t count) {
+    std::vector<float2> t(count);
+    const double M = std::ldexp(1.0, (int)M_log2);
+    for (uint32_t k = 0; k < count; ++k) {
+        const double a = 2.0 * M_PI * (double)((uint64_t)k * step) / M;
+        t[k] = make_float2((float)std::cos(a), (float)-std::sin(a));
+    }
+    return t;
+}
+
+int upload(tcfft_plan_s* p, const std::vector<float2>& h, float2** dst) {
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, h.size() * sizeof(float2));
+    if (e != hipSuccess) return hip_fail(e);
+    p->allocs.push_back(d);
+    e = hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e);
+    *dst = (float2*)d;
+    return TCFFT_SUCCESS;
+}
+
+void free_plan(tcfft_plan_s* p) {
+    for (void* a : p->allocs) (void)hipFree(a);
+    if (p->workspace) (void)hipFree(p->workspace);
+    p->magic = 0;
+    delete p;
+}
+
+int finish_plan(tcfft_plan_s* p) {
+    hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) return hip_fail(e);
+    int rc = upload(p, host_table(12, 1, 4096), &p->w4096);
+    if (rc) return rc;
+    for (Pass& ps : p->lay.passes) {
+        if (!ps.has_epi) continue;
+        const uint32_t M = ps.epi_M_log2, lo = ps.epi.lo_bits;
+        auto it = p->epi_tables.find(M);
+        if (it == p->epi_tables.end()) {
+            float2 *hi_d = nullptr, *lo_d = nullptr;
+            rc = upload(p, host_table(M, 1ull << lo, 1u << (M - lo)), &hi_d);
+            if (rc) return rc;
+            rc = upload(p, host_table(M, 1, 1u << lo), &lo_d);
+            if (rc) return rc;
+            it = p->epi_tables.emplace(M, std::make_pair(hi_d, lo_d)).first;
+        }
+        ps.epi.hi = it->second.first;
+        ps.epi.lo = it->second.second;
+    }
+    return TCFFT_SUCCESS;
+}
+
+int create_common(tcfft_plan* plan, const Layout& lay) {
+    tcfft_plan_s* p = new (std::nothrow) tcfft_plan_s;
+    if (!p) return TCFFT_ALLOC_FAILED;
+    p->lay = lay;
+    p->total = lay.dims == 1 ? (size_t)lay.n * lay.batch : (size_t)lay.nx * lay.ny * lay.batch;
+    int rc = finish_plan(p);
+    if (rc) {
+        free_plan(p);
+        return rc;
+    }
+    *plan = p;
+    return TCFFT_SUCCESS;
+}
+
+bool valid(tcfft_plan p) { return p && p->magic == kMagic; }
+
+int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t first = 0, size_t count = ~(size_t)0) {
+    const size_t end = count > p->lay.passes.size() - first ? p->lay.passes.size() : first + count;
+    for (size_t i = first; i < end; ++i) {
+        const Pass& ps = p->lay.passes[i];
+        const EpiParams* epi = ps.has_epi ? &ps.epi : nullptr;
+        hipError_t e;
+        if (ps.kind == 0) e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, st);
+        else e = tcfft::launch_cols(ps.radix, buf, ps.s_log2, ps.outer, sign, p->w4096, epi, st);
+        if (e != hipSuccess) return hip_fail(e);
+    }
+    return TCFFT_SUCCESS;
+}
+
+tcfft::RevSpec rev_spec(const std::vector<int>& rad) {
+    tcfft::RevSpec s{};
+    s.n_digits = (int)rad.size();
+    for (size_t i = 0; i < rad.size() && i < 12; ++i) s.radix[i] = rad[i];
+    return s;
+}
+
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev) {
+        int cur = 0;
+        if (hipGetDevice(&cur) == hipSuccess && cur != dev && hipSetDevice(dev) == hipSuccess) prev = cur;
+    }
+    ~DeviceGuard() {
+        if (prev >= 0) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int tcfft_version(void) { return 100; }
+
+const char* tcfft_status_string(int status) {
+    switch (status) {
+    case TCFFT_SUCCESS: return "TCFFT_SUCCESS";
+    case TCFFT_INVALID_SIZE: return "TCFFT_INVALID_SIZE";
+    case TCFFT_INVALID_BATCH: return "TCFFT_INVALID_BATCH";
+    case TCFFT_INVALID_POINTER: return "TCFFT_INVALID_POINTER";
+    case TCFFT_INVALID_PLAN: return "TCFFT_INVALID_PLAN";
+    case TCFFT_INVALID_VALUE: return "TCFFT_INVALID_VALUE";
+    case TCFFT_HIP_ERROR: return "TCFFT_HIP_ERROR";
+    case TCFFT_ALLOC_FAILED: return "TCFFT_ALLOC_FAILED";
+    default: return "TCFFT_UNKNOWN_STATUS";
+    }
+}
+
+int tcfft_last_hip_error(void) { return g_last_hip_error; }
+
+int tcfft_plan_create_1d(tcfft_plan* plan, int n, int n_batch) {
+    if (!plan) return TCFFT_INVALID_POINTER;
+    *plan = nullptr;
+    Layout lay;
+    int rc = build_layout_1d(n, n_batch, lay);
+    if (rc) return rc;
+    return create_common(plan, lay);
+}
+
+int tcfft_plan_create_2d(tcfft_plan* plan, int nx, int ny, int n_batch) {
+    if (!plan) return TCFFT_INVALID_POINTER;
+    *plan = nullptr;
+    Layout lay;
+    int rc = build_layout_2d(nx, ny, n_batch, lay);
+    if (rc) return rc;
+    return create_common(plan, lay);
+}
+
+int tcfft_plan_destroy(tcfft_plan plan) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    DeviceGuard guard(plan->device);
+    free_plan(plan);
+    return TCFFT_SUCCESS;
+}
+
+static int copy_radices(const std::vector<int>& rad, int* radices, int capacity, int* n_radices) {
+    if (!n_radices || (capacity > 0 && !radices)) return TCFFT_INVALID_POINTER;
+    *n_radices = (int)rad.size();
+    if (capacity < (int)rad.size()) return TCFFT_INVALID_VALUE;
+    for (size_t i = 0; i < rad.size(); ++i) radices[i] = rad[i];
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_plan_radices(tcfft_plan plan, int axis, int* radices, int capacity, int* n_radices) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    const Layout& L = plan->lay;
+    if (L.dims == 1) {
+        if (axis != 0) return TCFFT_INVALID_VALUE;
+        return copy_radices(L.rad_y, radices, capacity, n_radices);
+    }
+    if (axis == 0) return copy_radices(L.rad_x, radices, capacity, n_radices);
+    if (axis == 1) return copy_radices(L.rad_y, radices, capacity, n_radices);
+    return TCFFT_INVALID_VALUE;
+}
+
+int tcfft_query_1d(int n, int n_batch, int* radices, int capacity, int* n_radices, int* n_passes) {
+    Layout lay;
+    int rc = build_layout_1d(n, n_batch, lay);
+    if (rc) return rc;
+    if (n_passes) *n_passes = (int)lay.passes.size();
+    return copy_radices(lay.rad_y, radices, capacity, n_radices);
+}
+
+int tcfft_query_2d(int nx, int ny, int n_batch, int* radices_x, int* n_radices_x, int* radices_y, int* n_radices_y,
+                   int* n_passes, int capacity) {
+    Layout lay;
+    int rc = build_layout_2d(nx, ny, n_batch, lay);
+    if (rc) return rc;
+    if (n_passes) *n_passes = (int)lay.passes.size();
+    rc = copy_radices(lay.rad_x, radices_x, capacity, n_radices_x);
+    if (rc) return rc;
+    return copy_radices(lay.rad_y, radices_y, capacity, n_radices_y);
+}
+
+int tcfft_exec(tcfft_plan plan, void* data, int direction, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
+    if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
+    DeviceGuard guard(plan->device);
+    return run_passes(plan, (uint32_t*)data, direction, (hipStream_t)stream);
+}
+
+int tcfft_plan_num_passes(tcfft_plan plan, int* n_passes) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!n_passes) return TCFFT_INVALID_POINTER;
+    *n_passes = (int)plan->lay.passes.size();
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_exec_passes(tcfft_plan plan, void* data, int direction, int first, int count, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
+    if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
+    const int np = (int)plan->lay.passes.size();
+    if (first < 0 || count < 0 || first + count > np) return TCFFT_INVALID_VALUE;
+    DeviceGuard guard(plan->device);
+    return run_passes(plan, (uint32_t*)data, direction, (hipStream_t)stream, (size_t)first, (size_t)count);
+}
+
+int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!in || !out || ((uintptr_t)in & 15u) || ((uintptr_t)out & 15u)) return TCFFT_INVALID_POINTER;
+    if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
+    DeviceGuard guard(plan->device);
+    hipStream_t st = (hipStream_t)stream;
+    const size_t bytes = plan->total * 4;
+    const uint32_t* src = (const uint32_t*)in;
+    if (in == out) {
+        if (!plan->workspace) {
+            hipError_t e = hipMalloc(&plan->workspace, bytes);
+            if (e != hipSuccess) {
+                plan->workspace = nullptr;
+                return hip_fail(e);
+            }
+            plan->workspace_bytes = bytes;
+        }
+        hipError_t e = hipMemcpyAsync(plan->workspace, in, bytes, hipMemcpyDeviceToDevice, st);
+        if (e != hipSuccess) return hip_fail(e);
+        src = (const uint32_t*)plan->workspace;
+    }
+    const Layout& L = plan->lay;
+    hipError_t e;
+    if (L.dims == 1)
+        e = tcfft::launch_gather_rev_1d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.n), (uint32_t)plan->total,
+                                        rev_spec(L.rad_y), st);
+    else
+        e = tcfft::launch_gather_rev_2d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.nx),
+                                        (uint32_t)ilog2((uint32_t)L.ny), (uint32_t)plan->total, rev_spec(L.rad_x),
+                                        rev_spec(L.rad_y), st);
+    if (e != hipSuccess) return hip_fail(e);
+    return run_passes(plan, (uint32_t*)out, direction, st);
+}
+
+int tcfft_gen_rev(int n, const int* radices, int n_radices, int* rev) {
+    if (!radices || !rev) return TCFFT_INVALID_POINTER;
+    if (n_radices < 0 || n_radices > 64) return TCFFT_INVALID_VALUE;
+    long prod = 1;
+    for (int i = 0; i < n_radices; ++i) {
+        if (radices[i] < 2) return TCFFT_INVALID_VALUE;
+        prod *= radices[i];
+    }
+    if (prod != n) return TCFFT_INVALID_SIZE;
+    for (int p = 0; p < n; ++p) {
+        int q = p, idx = 0;
+        int digits[64];
+        for (int i = 0; i < n_radices; ++i) {
+            digits[i] = q % radices[i];
+            q /= radices[i];
+        }
+        for (int i = 0; i < n_radices; ++i) idx = idx * radices[i] + digits[i];
+        rev[p] = idx;
+    }
+    return TCFFT_SUCCESS;
+}
+
+}  // extern "C"
