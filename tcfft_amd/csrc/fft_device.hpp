@@ -92,6 +92,29 @@ __device__ __forceinline__ Dft16Op make_dft16(int lane, int sign) {
     return op;
 }
 
+// Operand of a DFT-16 whose output row f is multiplied by scale (a complex
+// FP32 value, here passed as the one for this lane's row f = lane%16).
+// Used in the B-form, where the constant is the A operand and its row is the
+// output frequency: D = diag(scale) * F * X.
+__device__ __forceinline__ Dft16Op make_dft16_scaled(int lane, int sign, cf scale) {
+    const int f = lane & 15, g = lane >> 4;
+    Dft16Op op;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int n = 4 * g + j;
+        const int m = (n * f) & 15;
+        const float fr0 = cos16(m);
+        const float fi0 = sign < 0 ? -sin16(m) : sin16(m);
+        const float fr = scale.x * fr0 - scale.y * fi0;
+        const float fi = scale.x * fi0 + scale.y * fr0;
+        op.re[2 * j] = (_Float16)fr;
+        op.re[2 * j + 1] = (_Float16)(-fi);
+        op.im[2 * j] = (_Float16)fi;
+        op.im[2 * j + 1] = (_Float16)fr;
+    }
+    return op;
+}
+
 __device__ __forceinline__ f16x8 as_operand(u32 a, u32 b, u32 c, u32 d) {
     u32x4 v = {a, b, c, d};
     return __builtin_bit_cast(f16x8, v);
@@ -126,6 +149,27 @@ __device__ __forceinline__ void swap_l5(u32& x, u32& y) {
     auto r = __builtin_amdgcn_permlane32_swap(x, y, false, false);
     x = r[0];
     y = r[1];
+}
+
+// Register transpose used between the in-wave stages: with register index
+// (t, i) -- t = 4-bit tile, i = 2-bit word -- and lane group g = lane/16,
+// exchange (g, i) with t: afterwards (g, i) holds what t held and t holds
+// (old i in bits 0-1, old g in bits 2-3).  A rename plus two permlane rounds.
+__device__ __forceinline__ void transpose_gi_tile(u32 (&src)[16][4], u32 (&dst)[16][4]) {
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dst[(t & 12) | i][t & 3] = src[t][i];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+        if (!(t & 4))
+#pragma unroll
+            for (int i = 0; i < 4; ++i) swap_l4(dst[t][i], dst[t | 4][i]);
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+        if (!(t & 8))
+#pragma unroll
+            for (int i = 0; i < 4; ++i) swap_l5(dst[t][i], dst[t | 8][i]);
 }
 
 __device__ __forceinline__ void load4(u32* w, const u32* __restrict__ p) {

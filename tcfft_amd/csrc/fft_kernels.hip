@@ -351,6 +351,203 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
 }
 
 // ----------------------------------------------------------------------------
+// In-wave 4096-point DIT over the digits d0, d1, d2 of a 4096-element chunk.
+// In : d[t][j] = tile t (d2), lane (c16 = d1, g), word j (d0 = 4g + j).
+// Out: d[t][i] = tile t (k0), lane%16 = k1, (g, i) = k2, after post(t, i, v)
+//      is applied to every FP32 result of the last stage.
+// Stage-1 twiddle W_256^{k0*d1} is a per-lane constant; the stage-2 twiddle
+// W_4096^{(k0+16k1)*d2} is a per-lane recurrence over the tiles.  The
+// last stage uses the operand op2 (B-form), which may carry a row scale.
+template <class Post>
+__device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, const float2* __restrict__ w4096,
+                                        const Dft16Op& op, const Dft16Op& op2, Post post) {
+    const int c16 = lane & 15, g = lane >> 4;
+    const float sg = sign < 0 ? 1.f : -1.f;
+    cf tw1[4], step[4], rec[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tw1[i] = twiddle(w4096, ((c16 * (4 * g + i)) & 255) * 16, sg);
+        step[i] = twiddle(w4096, 16 * c16 + 4 * g + i, sg);
+        rec[i] = step[i];
+    }
+    // stage 0: DFT over d0 -> lane%16 = k0, (g,i) = d1
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        f32x4 re, im;
+        dft16_a(d[t], op, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
+    }
+    // stage 1: DFT over d1 -> lane%16 = k1, (g,i) = k0, tile = d2; twiddle W_4096^{c*d2}
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        f32x4 re, im;
+        dft16_a(d[t], op, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            cf v = {re[i], im[i]};
+            if (t) {
+                v = cmul(v, rec[i]);
+                rec[i] = cmul(rec[i], step[i]);
+            }
+            d[t][i] = pack(v);
+        }
+    }
+    u32 e[16][4];
+    transpose_gi_tile(d, e);  // (g,i) = d2, tile = k0
+    // stage 2 (B-form): columns k1, points d2 -> (g,i) = k2
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        f32x4 re, im;
+        dft16_b(e[t], op2, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[t][i] = pack(post(t, i, cf{re[i], im[i]}));
+    }
+}
+
+// ----------------------------------------------------------------------------
+// k_big<W>: one signal of N = 4096*W (W = 2, 4, 8, 16) per workgroup of W
+// waves, in a single HBM round trip.  Radix list [16, 16, 16, W]; position
+// p = d0 + 16*d1 + 256*d2 + 4096*w, wave w loads chunk w.
+//  phase 1: each wave does the 4096-point DIT of its chunk in registers and
+//           multiplies by the final-stage twiddle W_N^{w*c}, c = k0+16k1+256k2:
+//           W_N^{256*w*k2} is folded into the stage-2 operand (row scale),
+//           W_N^{w*(k0+16k1)} is a per-lane recurrence over the tiles.
+//  phase 2: LDS exchange (layout [w][sigma(c)], b128 both ways) and the
+//           radix-W stage over w (MFMA for W = 16, VALU for W = 2, 4, 8);
+//           output position c + 4096*k3.  Two rounds of 2048 columns for
+//           W >= 8 keep the exchange at W*8 KiB of LDS.
+// sigma keeps c bits 0-1, moves bits 4-7 to 2-5 and bits 2-3 to 6-7 so that
+// 16 adjacent lanes touch 256 contiguous LDS bytes on both sides.
+__device__ __forceinline__ u32 lds_sigma(u32 c) {
+    return (c & 3u) | (((c >> 4) & 15u) << 2) | (((c >> 2) & 3u) << 6) | (c & ~255u);
+}
+
+template <int W>
+struct BigCfg {
+    static constexpr int ROUNDS = W >= 8 ? 2 : 1;
+    static constexpr int COLS = 4096 / ROUNDS;  // columns exchanged per round
+    static constexpr int LDS_WORDS = W * COLS;
+    static constexpr u32 N = 4096u * W;
+};
+
+template <int W, bool EPI>
+__global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, int sign, const float2* __restrict__ w4096,
+                                                const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
+                                                u32 wn_lo_bits, EpiParams epi) {
+    using C = BigCfg<W>;
+    extern __shared__ u32 lds[];
+    const int lane = threadIdx.x & 63;
+    const int c16 = lane & 15, g = lane >> 4;
+    const int w = threadIdx.x >> 6;
+    const float sg = sign < 0 ? 1.f : -1.f;
+    u32* const sbase = buf + (size_t)blockIdx.x * C::N;
+
+    u32 d[16][4];
+    {
+        const u32* src = sbase + 4096 * w + 16 * c16 + 4 * g;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) load4(d[t], src + 256 * t);
+    }
+    // W_N^m for m < N from the two-level table
+    auto wn = [&](u32 m) -> cf {
+        m &= C::N - 1u;
+        return cmul(twiddle(wn_hi, m >> wn_lo_bits, sg), twiddle(wn_lo, m & ((1u << wn_lo_bits) - 1u), sg));
+    };
+    const Dft16Op op = make_dft16(lane, sign);
+    // stage-2 operand rows f = lane%16 scaled by W_N^{256*w*f}
+    const Dft16Op op2 = make_dft16_scaled(lane, sign, wn(256u * (u32)w * (u32)c16));
+    // per-(lane, tile) factor W_N^{w*(16*k1 + k0)}, k1 = lane%16, k0 = tile
+    cf fac = wn(16u * (u32)w * (u32)c16);
+    const cf fstep = wn((u32)w);
+    dit4096(d, lane, sign, w4096, op, op2, [&](int t, int i, cf v) -> cf {
+        if (i == 0 && t > 0) fac = cmul(fac, fstep);
+        return cmul(v, fac);
+    });
+    // d: tile = k0, lane%16 = k1, (g,i) = k2.  Writer layout: word = c bits 0-1
+    u32 e[16][4];
+    transpose_gi_tile(d, e);  // (g,i) = k0, tile = k2: c = (4g+i) + 16*c16 + 256*t
+
+#pragma unroll
+    for (int r = 0; r < C::ROUNDS; ++r) {
+        constexpr int TPR = 16 / C::ROUNDS;  // tiles per round
+        if (r) __syncthreads();              // previous round's reads are done
+#pragma unroll
+        for (int tt = 0; tt < TPR; ++tt) {
+            const u32 off = (u32)w * C::COLS + 4u * c16 + 64u * g + 256u * tt;
+            store4(lds + off, e[r * TPR + tt]);
+        }
+        __syncthreads();
+        if constexpr (W == 16) {
+            // reader wave w': columns c = j + 4h + 8(w'&1) + 16*c16 + 256(w'>>1) (+2048r)
+            u32 v[2][4][4];  // [h][i = point (w = 4g+i)][j]
+#pragma unroll
+            for (int h = 0; h < 2; ++h)
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const u32 off = (u32)(4 * g + i) * C::COLS + 4u * c16 + 64u * (h + 2 * (w & 1)) + 256u * (w >> 1);
+                    load4(v[h][i], lds + off);
+                }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                u32 o[4][4];  // [j][i' = k3 word]
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    u32 a[4] = {v[h][0][j], v[h][1][j], v[h][2][j], v[h][3][j]};
+                    f32x4 re, im;
+                    dft16_b(a, op, re, im);
+#pragma unroll
+                    for (int i = 0; i < 4; ++i) {
+                        cf x = {re[i], im[i]};
+                        if constexpr (EPI) {
+                            const u32 c = 2048u * r + j + 4u * h + 8u * (w & 1) + 16u * c16 + 256u * (w >> 1);
+                            const u32 pos = blockIdx.x * C::N + c + 4096u * (4u * g + i);
+                            x = cmul(x, epi_twiddle(epi, pos, sg));
+                        }
+                        o[j][i] = pack(x);
+                    }
+                }
+                const u32 c0 = 2048u * r + 4u * h + 8u * (w & 1) + 16u * c16 + 256u * (w >> 1);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    u32 q[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
+                    store4(sbase + c0 + 4096u * (4u * g + i), q);
+                }
+            }
+        } else {
+            // reader wave w': columns c = j + 4g + 16*c16 + 256*(q + QN*w') (+2048r)
+            constexpr int QN = C::COLS / (256 * W);
+#pragma unroll
+            for (int q = 0; q < QN; ++q) {
+                u32 v[W][4];
+#pragma unroll
+                for (int s = 0; s < W; ++s) load4(v[s], lds + (u32)s * C::COLS + 4u * lane + 256u * (q + QN * w));
+                u32 o[W][4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    cf x[W];
+#pragma unroll
+                    for (int s = 0; s < W; ++s) x[s] = unpack(v[s][j]);
+                    small_dft<W>(x, sign);
+#pragma unroll
+                    for (int s = 0; s < W; ++s) {
+                        cf y = x[s];
+                        if constexpr (EPI) {
+                            const u32 c = 2048u * r + j + 4u * g + 16u * c16 + 256u * (q + QN * w);
+                            y = cmul(y, epi_twiddle(epi, blockIdx.x * C::N + c + 4096u * s, sg));
+                        }
+                        o[s][j] = pack(y);
+                    }
+                }
+                const u32 c0 = 2048u * r + 4u * g + 16u * c16 + 256u * (q + QN * w);
+#pragma unroll
+                for (int s = 0; s < W; ++s) store4(sbase + c0 + 4096u * s, o[s]);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Host-side launchers (called by the pass driver).
 template <int T>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
@@ -405,6 +602,36 @@ hipError_t launch_cols(int R, u32* buf, u32 s_log2, u32 outer, int sign, const f
     case 64: return launch_cols_t<4>(buf, s_log2, outer, sign, w4096, epi, st);
     case 128: return launch_cols_t<8>(buf, s_log2, outer, sign, w4096, epi, st);
     case 256: return launch_cols_t<16>(buf, s_log2, outer, sign, w4096, epi, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int W>
+static hipError_t launch_big_t(u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
+                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, hipStream_t st) {
+    const size_t lds = BigCfg<W>::LDS_WORDS * sizeof(u32);
+    if (epi) {
+        auto k = k_big<W, true>;
+        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, sign, w4096, wn_hi, wn_lo, lo_bits, *epi);
+    } else {
+        auto k = k_big<W, false>;
+        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, sign, w4096, wn_hi, wn_lo, lo_bits,
+                           EpiParams{});
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_big(int W, u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
+                      const float2* wn_lo, u32 lo_bits, const EpiParams* epi, hipStream_t st) {
+    switch (W) {
+    case 2: return launch_big_t<2>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
+    case 4: return launch_big_t<4>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
+    case 8: return launch_big_t<8>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
+    case 16: return launch_big_t<16>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -23,6 +23,11 @@ struct EpiParams {
 hipError_t launch_rows(int T, uint32_t* buf, uint32_t n_tiles, int sign, const float2* w4096,
                        const EpiParams* epi, hipStream_t st);
 
+// Single-pass transform of N = 4096*W (W = 2, 4, 8, 16) per workgroup of W
+// waves.  wn_hi/wn_lo: two-level table of W_N (lo_bits split).
+hipError_t launch_big(int W, uint32_t* buf, uint32_t n_signals, int sign, const float2* w4096, const float2* wn_hi,
+                      const float2* wn_lo, uint32_t lo_bits, const EpiParams* epi, hipStream_t st);
+
 // Strided pass of radix R (2..256) at element step 2^s_log2 over `outer` blocks.
 hipError_t launch_cols(int R, uint32_t* buf, uint32_t s_log2, uint32_t outer, int sign, const float2* w4096,
                        const EpiParams* epi, hipStream_t st);

@@ -28,14 +28,17 @@ constexpr uint32_t kMagic = 0x7cff7a11u;
 thread_local int g_last_hip_error = 0;
 
 struct Pass {
-    int kind = 0;          // 0: contiguous rows, 1: strided columns
-    int radix = 0;         // rows: T (line = 256*T); columns: R
-    uint32_t n_tiles = 0;  // rows
+    int kind = 0;          // 0: contiguous rows, 1: strided columns, 2: single-pass big lines
+    int radix = 0;         // rows: T (line = 256*T); columns: R; big: W (line = 4096*W)
+    uint32_t n_tiles = 0;  // rows: 256-element tiles; big: number of lines
     uint32_t s_log2 = 0;   // columns: element step
     uint32_t outer = 0;    // columns: number of [R][s] blocks
     bool has_epi = false;
     EpiParams epi{};
     uint32_t epi_M_log2 = 0;  // table size to bind at upload time
+    uint32_t big_M_log2 = 0;  // big: W_L table (L = 4096*W)
+    const float2* big_hi = nullptr;
+    const float2* big_lo = nullptr;
 };
 
 struct Layout {  // host-only result of the plan builder
@@ -85,18 +88,31 @@ void set_epi(Pass& p, uint32_t s_log2_axis, int R_next, uint32_t shift) {
     p.epi.lo_bits = p.epi_M_log2 / 2;
 }
 
-// Passes for contiguous transforms of length n over `lines` lines.
+// Passes for contiguous transforms of length n over `lines` lines.  The first
+// pass covers L = min(n, 65536) contiguous elements: one wave per line for
+// L <= 4096 (k_rows), one workgroup of L/4096 waves for larger L (k_big,
+// a single HBM round trip).
 int build_axis_rows(int n, long lines, std::vector<int>& rad, std::vector<Pass>& passes) {
     const int nb = ilog2((uint32_t)n);
-    const int L = n < 4096 ? n : 4096;
-    const int T = L / 256;
+    const int L = n < 65536 ? n : 65536;
     Pass first;
-    first.kind = 0;
-    first.radix = T;
-    first.n_tiles = (uint32_t)((long)n * lines / 256);
     rad.push_back(16);
     rad.push_back(16);
-    if (T > 1) rad.push_back(T);
+    if (L <= 4096) {
+        const int T = L / 256;
+        first.kind = 0;
+        first.radix = T;
+        first.n_tiles = (uint32_t)((long)n * lines / 256);
+        if (T > 1) rad.push_back(T);
+    } else {
+        const int W = L / 4096;
+        first.kind = 2;
+        first.radix = W;
+        first.n_tiles = (uint32_t)((long)n * lines / L);
+        first.big_M_log2 = (uint32_t)ilog2((uint32_t)L);
+        rad.push_back(16);
+        rad.push_back(W);
+    }
     passes.push_back(first);
     uint32_t s_log2 = (uint32_t)ilog2((uint32_t)L);
     for (int R : split_bits(nb - (int)s_log2)) {
@@ -205,25 +221,42 @@ void free_plan(tcfft_plan_s* p) {
     delete p;
 }
 
+// Two-level table of W_M, M = 2^M_log2: hi[k] = W_M^{k*2^lo}, lo[k] = W_M^k,
+// lo = floor(M_log2 / 2) (the split every kernel uses).
+int table_pair(tcfft_plan_s* p, uint32_t M, std::pair<float2*, float2*>& out) {
+    auto it = p->epi_tables.find(M);
+    if (it == p->epi_tables.end()) {
+        const uint32_t lo = M / 2;
+        float2 *hi_d = nullptr, *lo_d = nullptr;
+        int rc = upload(p, host_table(M, 1ull << lo, 1u << (M - lo)), &hi_d);
+        if (rc) return rc;
+        rc = upload(p, host_table(M, 1, 1u << lo), &lo_d);
+        if (rc) return rc;
+        it = p->epi_tables.emplace(M, std::make_pair(hi_d, lo_d)).first;
+    }
+    out = it->second;
+    return TCFFT_SUCCESS;
+}
+
 int finish_plan(tcfft_plan_s* p) {
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) return hip_fail(e);
     int rc = upload(p, host_table(12, 1, 4096), &p->w4096);
     if (rc) return rc;
     for (Pass& ps : p->lay.passes) {
-        if (!ps.has_epi) continue;
-        const uint32_t M = ps.epi_M_log2, lo = ps.epi.lo_bits;
-        auto it = p->epi_tables.find(M);
-        if (it == p->epi_tables.end()) {
-            float2 *hi_d = nullptr, *lo_d = nullptr;
-            rc = upload(p, host_table(M, 1ull << lo, 1u << (M - lo)), &hi_d);
+        if (ps.kind == 2) {
+            std::pair<float2*, float2*> t;
+            rc = table_pair(p, ps.big_M_log2, t);
             if (rc) return rc;
-            rc = upload(p, host_table(M, 1, 1u << lo), &lo_d);
-            if (rc) return rc;
-            it = p->epi_tables.emplace(M, std::make_pair(hi_d, lo_d)).first;
+            ps.big_hi = t.first;
+            ps.big_lo = t.second;
         }
-        ps.epi.hi = it->second.first;
-        ps.epi.lo = it->second.second;
+        if (!ps.has_epi) continue;
+        std::pair<float2*, float2*> t;
+        rc = table_pair(p, ps.epi_M_log2, t);
+        if (rc) return rc;
+        ps.epi.hi = t.first;
+        ps.epi.lo = t.second;
     }
     return TCFFT_SUCCESS;
 }
@@ -251,6 +284,9 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
         const EpiParams* epi = ps.has_epi ? &ps.epi : nullptr;
         hipError_t e;
         if (ps.kind == 0) e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, st);
+        else if (ps.kind == 2)
+            e = tcfft::launch_big(ps.radix, buf, ps.n_tiles, sign, p->w4096, ps.big_hi, ps.big_lo, ps.big_M_log2 / 2,
+                                  epi, st);
         else e = tcfft::launch_cols(ps.radix, buf, ps.s_log2, ps.outer, sign, p->w4096, epi, st);
         if (e != hipSuccess) return hip_fail(e);
     }
