@@ -46,6 +46,8 @@ def parse():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--calib", action="store_true",
+                    help="also run 5 device-to-device copies of one 64 MiB buffer (counter calibration kernel)")
     return ap.parse_args()
 
 
@@ -108,6 +110,11 @@ def main():
         b.copy_(host)
     torch.cuda.synchronize()
 
+    if args.calib:
+        for _ in range(5):
+            bufs[1].copy_(bufs[0])
+        torch.cuda.synchronize()
+
     # accuracy of one exec vs FP64 (numpy pocketfft) on a few signals
     check = bufs[0]
     plan.exec(check.data_ptr(), tcfft_amd.FORWARD, sptr)
@@ -120,36 +127,55 @@ def main():
     rel_l2 = float(np.linalg.norm(gotc - ref) / np.linalg.norm(ref))
     check.copy_(host)
 
-    # whole-exec timing
-    for i in range(args.warmup):
-        plan.exec(bufs[i % ROTATE].data_ptr(), tcfft_amd.FORWARD, sptr)
-    torch.cuda.synchronize()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    t_wall0 = time.perf_counter()
-    e0.record(stream)
-    for i in range(args.steps):
-        plan.exec(bufs[i % ROTATE].data_ptr(), tcfft_amd.FORWARD, sptr)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    t_wall = time.perf_counter() - t_wall0
-    ms = e0.elapsed_time(e1) / args.steps
+    def timed(launch, reps, graph):
+        """Mean ms per launch() over reps, bracketed by synchronize; with
+        graph=True the reps launches are captured once into a hipGraph and
+        replayed (no host launch overhead inside the timed region)."""
+        if graph:
+            g = torch.cuda.CUDAGraph()
+            cs = torch.cuda.Stream()
+            cs.wait_stream(torch.cuda.current_stream())
+            with torch.cuda.graph(g, stream=cs):
+                for i in range(reps):
+                    launch(i, cs.cuda_stream)
+            g.replay()  # untimed replay (replay() runs on the current stream)
+            torch.cuda.synchronize()
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(torch.cuda.current_stream())
+            g.replay()
+            b.record(torch.cuda.current_stream())
+            torch.cuda.synchronize()
+            return a.elapsed_time(b) / reps
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        a.record(stream)
+        for i in range(reps):
+            launch(i, sptr)
+        b.record(stream)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / reps
 
-    # per-pass timing (each pass kernel alone, same rotation, same stream)
+    def exec_step(i, st):
+        plan.exec(bufs[i % ROTATE].data_ptr(), tcfft_amd.FORWARD, st)
+
+    # warmup (eager), then the timed K steps: eager and graph-replayed
+    for i in range(args.warmup):
+        exec_step(i, sptr)
+    torch.cuda.synchronize()
+    t_wall0 = time.perf_counter()
+    ms_eager = timed(exec_step, args.steps, graph=False)
+    ms = timed(exec_step, args.steps, graph=True)
+    t_wall = time.perf_counter() - t_wall0
+
+    # per-pass timing (each pass kernel alone, graph-replayed on its stream)
     npass = plan.num_passes
     per_pass = []
     for p in range(npass):
-        reps = max(20, args.steps // 2)
-        for i in range(5):
-            plan.exec_passes(bufs[i % ROTATE].data_ptr(), p, 1, tcfft_amd.FORWARD, sptr)
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        a.record(stream)
-        for i in range(reps):
-            plan.exec_passes(bufs[i % ROTATE].data_ptr(), p, 1, tcfft_amd.FORWARD, sptr)
-        b.record(stream)
-        torch.cuda.synchronize()
-        per_pass.append(a.elapsed_time(b) / reps)
+        def pass_step(i, st, p=p):
+            plan.exec_passes(bufs[i % ROTATE].data_ptr(), p, 1, tcfft_amd.FORWARD, st)
+        per_pass.append(timed(pass_step, max(20, args.steps), graph=True))
     dom = int(np.argmax(per_pass))
     alg_bytes = 8.0 * n * batch
     achieved = alg_bytes / (per_pass[dom] * 1e-3) / 1e9
@@ -165,6 +191,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": round(ms, 5),
+        "ms_per_step_eager": round(ms_eager, 5),
+        "timing": "hipGraph replay of the K timed steps (eager launch loop in ms_per_step_eager)",
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,

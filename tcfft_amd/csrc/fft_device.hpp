@@ -172,6 +172,19 @@ __device__ __forceinline__ void transpose_gi_tile(u32 (&src)[16][4], u32 (&dst)[
             for (int i = 0; i < 4; ++i) swap_l5(dst[t][i], dst[t | 8][i]);
 }
 
+// Streaming (non-temporal) variants for data touched once per pass.
+__device__ __forceinline__ void load4_nt(u32* w, const u32* __restrict__ p) {
+    u32x4 v = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(p));
+    w[0] = v[0];
+    w[1] = v[1];
+    w[2] = v[2];
+    w[3] = v[3];
+}
+__device__ __forceinline__ void store4_nt(u32* __restrict__ p, const u32* w) {
+    u32x4 v = {w[0], w[1], w[2], w[3]};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(p));
+}
+
 __device__ __forceinline__ void load4(u32* w, const u32* __restrict__ p) {
     u32x4 v = *reinterpret_cast<const u32x4*>(p);
     w[0] = v[0];

@@ -19,6 +19,8 @@
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
 
+#include <mutex>
+
 namespace tcfft {
 
 // Epilogue twiddle for the next pass: element at position P (within the
@@ -39,13 +41,69 @@ __device__ __forceinline__ cf epi_twiddle(const EpiParams& e, u32 gpos, float sg
 }
 
 // ----------------------------------------------------------------------------
+// In-wave 4096-point DIT over the digits d0, d1, d2 of a 4096-element chunk.
+// In : d[t][j] = tile t (d2), lane (c16 = d1, g), word j (d0 = 4g + j).
+// Out: d[t][i] = tile t (k0), lane%16 = k1, (g, i) = k2, after post(t, i, v)
+//      is applied to every FP32 result of the last stage.
+// Stage-1 twiddle W_256^{k0*d1} is a per-lane constant; the stage-2 twiddle
+// W_4096^{(k0+16k1)*d2} is a per-lane recurrence over the tiles.  The
+// last stage uses the operand op2 (B-form), which may carry a row scale.
+template <class Post>
+__device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, const float2* __restrict__ w4096,
+                                        const Dft16Op& op, const Dft16Op& op2, Post post) {
+    const int c16 = lane & 15, g = lane >> 4;
+    const float sg = sign < 0 ? 1.f : -1.f;
+    cf tw1[4], step[4], rec[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tw1[i] = twiddle(w4096, ((c16 * (4 * g + i)) & 255) * 16, sg);
+        step[i] = twiddle(w4096, 16 * c16 + 4 * g + i, sg);
+        rec[i] = step[i];
+    }
+    // stage 0: DFT over d0 -> lane%16 = k0, (g,i) = d1
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        f32x4 re, im;
+        dft16_a(d[t], op, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
+    }
+    // stage 1: DFT over d1 -> lane%16 = k1, (g,i) = k0, tile = d2; twiddle W_4096^{c*d2}
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        f32x4 re, im;
+        dft16_a(d[t], op, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            cf v = {re[i], im[i]};
+            if (t) {
+                v = cmul(v, rec[i]);
+                rec[i] = cmul(rec[i], step[i]);
+            }
+            d[t][i] = pack(v);
+        }
+    }
+    u32 e[16][4];
+    transpose_gi_tile(d, e);  // (g,i) = d2, tile = k0
+    // stage 2 (B-form): columns k1, points d2 -> (g,i) = k2
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        f32x4 re, im;
+        dft16_b(e[t], op2, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[t][i] = pack(post(t, i, cf{re[i], im[i]}));
+    }
+}
+
+// ----------------------------------------------------------------------------
 // k_rows<T>: lines of L = 256*T contiguous elements, digit-reversed for the
 // radix list [16, 16] (+ [T] when T > 1).  Position in a line:
 // p = d0 + 16*d1 + 256*d2.  Wave layout at load: tile t = 256-element block,
 // lane (c16 = d1, g) holds d0 = 4g..4g+3 in its four words (one 16-byte load).
-template <int T, bool EPI>
-__global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles, int sign,
+template <int T, bool EPI, int SIGN>
+__global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles,
                                               const float2* __restrict__ w4096, EpiParams epi) {
+    constexpr int sign = SIGN;
     const int lane = threadIdx.x & 63;
     const int c16 = lane & 15, g = lane >> 4;
     const u32 wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -63,18 +121,20 @@ __global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles
     u32* const base = buf + (size_t)tile0 * 256 + 16 * c16 + 4 * g;
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-        if ((u32)t < ntl) load4(d[t], base + t * 256);
+        if ((u32)t < ntl) load4_nt(d[t], base + t * 256);
         else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
     }
 
-    // Stage 0: DFT over d0 (A-form) -> lane%16 = k0, rows (g,i) = d1.
-    // Twiddle W_256^{k0*d1} is a per-lane constant.
+    // Stage 0 (T < 16; dit4096 does its own): DFT over d0 (A-form) ->
+    // lane%16 = k0, rows (g,i) = d1.  Twiddle W_256^{k0*d1} is a per-lane constant.
+    if constexpr (T < 16) {
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
-        dft16_a(d[t], op, re, im);
+        for (int t = 0; t < 16; ++t) {
+            f32x4 re, im;
+            dft16_a(d[t], op, re, im);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
+            for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
+        }
     }
 
     if constexpr (T == 1) {
@@ -137,70 +197,20 @@ __global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles
             }
         }
     } else {
-        // T == 16: one 4096-point line per wave.
-        // Stage 1 -> lane%16 = k1, rows (g,i) = k0, tile = d2; then the
-        // stage-2 twiddle W_4096^{(k0+16k1)*d2} in FP32 before rounding.
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            f32x4 re, im;
-            dft16_a(d[t], op, re, im);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const u32 c = 16u * c16 + 4u * g + i;
-                cf v = {re[i], im[i]};
-                if (t) v = cmul(v, twiddle(w4096, (c * t) & 4095u, sg));
-                d[t][i] = pack(v);
+        // T == 16: one 4096-point line per wave (shared in-wave DIT).
+        dit4096(d, lane, sign, w4096, op, op, [&](int t, int i, cf v) -> cf {
+            if constexpr (EPI) {
+                const u32 k = (u32)t + 16u * c16 + 256u * (4u * g + i);
+                v = cmul(v, epi_twiddle(epi, tile0 * 256u + k, sg));
             }
-        }
-        // Register transpose: (g,i) = k0, tile = d2  ->  (g,i) = d2, tile = k0.
+            return v;
+        });
+        // (g,i) = k2, tile = k0 -> (g,i) = k0, tile = k2: four adjacent words per lane
         u32 e[16][4];
+        transpose_gi_tile(d, e);
 #pragma unroll
         for (int t = 0; t < 16; ++t)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) e[(t & 12) | i][t & 3] = d[t][i];
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (!(t & 4))
-#pragma unroll
-                for (int i = 0; i < 4; ++i) swap_l4(e[t][i], e[t | 4][i]);
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (!(t & 8))
-#pragma unroll
-                for (int i = 0; i < 4; ++i) swap_l5(e[t][i], e[t | 8][i]);
-        // Stage 2 (B-form): columns k1 = lane%16, points d2 = (g,i) -> rows (g,i) = k2.
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            f32x4 re, im;
-            dft16_b(e[t], op, re, im);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                cf v = {re[i], im[i]};
-                if constexpr (EPI) {
-                    const u32 k = (u32)t + 16u * c16 + 256u * (4u * g + i);
-                    v = cmul(v, epi_twiddle(epi, tile0 * 256u + k, sg));
-                }
-                e[t][i] = pack(v);
-            }
-        }
-        // Register transpose for the store: (g,i) = k2, tile = k0 -> (g,i) = k0, tile = k2.
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) d[(t & 12) | i][t & 3] = e[t][i];
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (!(t & 4))
-#pragma unroll
-                for (int i = 0; i < 4; ++i) swap_l4(d[t][i], d[t | 4][i]);
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (!(t & 8))
-#pragma unroll
-                for (int i = 0; i < 4; ++i) swap_l5(d[t][i], d[t | 8][i]);
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if ((u32)t < ntl) store4(base + t * 256, d[t]);
+            if ((u32)t < ntl) store4_nt(base + t * 256, e[t]);
     }
 }
 
@@ -351,61 +361,6 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
 }
 
 // ----------------------------------------------------------------------------
-// In-wave 4096-point DIT over the digits d0, d1, d2 of a 4096-element chunk.
-// In : d[t][j] = tile t (d2), lane (c16 = d1, g), word j (d0 = 4g + j).
-// Out: d[t][i] = tile t (k0), lane%16 = k1, (g, i) = k2, after post(t, i, v)
-//      is applied to every FP32 result of the last stage.
-// Stage-1 twiddle W_256^{k0*d1} is a per-lane constant; the stage-2 twiddle
-// W_4096^{(k0+16k1)*d2} is a per-lane recurrence over the tiles.  The
-// last stage uses the operand op2 (B-form), which may carry a row scale.
-template <class Post>
-__device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, const float2* __restrict__ w4096,
-                                        const Dft16Op& op, const Dft16Op& op2, Post post) {
-    const int c16 = lane & 15, g = lane >> 4;
-    const float sg = sign < 0 ? 1.f : -1.f;
-    cf tw1[4], step[4], rec[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        tw1[i] = twiddle(w4096, ((c16 * (4 * g + i)) & 255) * 16, sg);
-        step[i] = twiddle(w4096, 16 * c16 + 4 * g + i, sg);
-        rec[i] = step[i];
-    }
-    // stage 0: DFT over d0 -> lane%16 = k0, (g,i) = d1
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
-        dft16_a(d[t], op, re, im);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
-    }
-    // stage 1: DFT over d1 -> lane%16 = k1, (g,i) = k0, tile = d2; twiddle W_4096^{c*d2}
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
-        dft16_a(d[t], op, re, im);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            cf v = {re[i], im[i]};
-            if (t) {
-                v = cmul(v, rec[i]);
-                rec[i] = cmul(rec[i], step[i]);
-            }
-            d[t][i] = pack(v);
-        }
-    }
-    u32 e[16][4];
-    transpose_gi_tile(d, e);  // (g,i) = d2, tile = k0
-    // stage 2 (B-form): columns k1, points d2 -> (g,i) = k2
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
-        dft16_b(e[t], op2, re, im);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) d[t][i] = pack(post(t, i, cf{re[i], im[i]}));
-    }
-}
-
-// ----------------------------------------------------------------------------
 // k_big<W>: one signal of N = 4096*W (W = 2, 4, 8, 16) per workgroup of W
 // waves, in a single HBM round trip.  Radix list [16, 16, 16, W]; position
 // p = d0 + 16*d1 + 256*d2 + 4096*w, wave w loads chunk w.
@@ -417,11 +372,11 @@ __device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, con
 //           radix-W stage over w (MFMA for W = 16, VALU for W = 2, 4, 8);
 //           output position c + 4096*k3.  Two rounds of 2048 columns for
 //           W >= 8 keep the exchange at W*8 KiB of LDS.
-// sigma keeps c bits 0-1, moves bits 4-7 to 2-5 and bits 2-3 to 6-7 so that
-// 16 adjacent lanes touch 256 contiguous LDS bytes on both sides.
-__device__ __forceinline__ u32 lds_sigma(u32 c) {
-    return (c & 3u) | (((c >> 4) & 15u) << 2) | (((c >> 2) & 3u) << 6) | (c & ~255u);
-}
+// sigma(c) = c ^ ((c >> 4) & 15) << 2 is a bijection that keeps 16-lane
+// groups conflict-free both for writers (lane%16 = c bits 4-7) and for
+// readers (lane%16 = c bits 2-5); readers own 4 adjacent columns per lane so
+// the final global stores are 256-byte runs per 16 lanes.
+__device__ __forceinline__ u32 lds_sigma(u32 c) { return c ^ (((c >> 4) & 15u) << 2); }
 
 template <int W>
 struct BigCfg {
@@ -431,10 +386,11 @@ struct BigCfg {
     static constexpr u32 N = 4096u * W;
 };
 
-template <int W, bool EPI>
-__global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, int sign, const float2* __restrict__ w4096,
+template <int W, bool EPI, int SIGN>
+__global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, const float2* __restrict__ w4096,
                                                 const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
                                                 u32 wn_lo_bits, EpiParams epi) {
+    constexpr int sign = SIGN;
     using C = BigCfg<W>;
     extern __shared__ u32 lds[];
     const int lane = threadIdx.x & 63;
@@ -447,7 +403,7 @@ __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, int sign,
     {
         const u32* src = sbase + 4096 * w + 16 * c16 + 4 * g;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) load4(d[t], src + 256 * t);
+        for (int t = 0; t < 16; ++t) load4_nt(d[t], src + 256 * t);
     }
     // W_N^m for m < N from the two-level table
     auto wn = [&](u32 m) -> cf {
@@ -474,18 +430,18 @@ __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, int sign,
         if (r) __syncthreads();              // previous round's reads are done
 #pragma unroll
         for (int tt = 0; tt < TPR; ++tt) {
-            const u32 off = (u32)w * C::COLS + 4u * c16 + 64u * g + 256u * tt;
+            const u32 off = (u32)w * C::COLS + lds_sigma(4u * g + 16u * c16 + 256u * tt);
             store4(lds + off, e[r * TPR + tt]);
         }
         __syncthreads();
         if constexpr (W == 16) {
-            // reader wave w': columns c = j + 4h + 8(w'&1) + 16*c16 + 256(w'>>1) (+2048r)
+            // reader wave w': columns c = j + 4*c16 + 64h + 128w' (+2048r)
             u32 v[2][4][4];  // [h][i = point (w = 4g+i)][j]
 #pragma unroll
             for (int h = 0; h < 2; ++h)
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const u32 off = (u32)(4 * g + i) * C::COLS + 4u * c16 + 64u * (h + 2 * (w & 1)) + 256u * (w >> 1);
+                    const u32 off = (u32)(4 * g + i) * C::COLS + lds_sigma(4u * c16 + 64u * h + 128u * w);
                     load4(v[h][i], lds + off);
                 }
 #pragma unroll
@@ -500,28 +456,29 @@ __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, int sign,
                     for (int i = 0; i < 4; ++i) {
                         cf x = {re[i], im[i]};
                         if constexpr (EPI) {
-                            const u32 c = 2048u * r + j + 4u * h + 8u * (w & 1) + 16u * c16 + 256u * (w >> 1);
+                            const u32 c = 2048u * r + j + 4u * c16 + 64u * h + 128u * w;
                             const u32 pos = blockIdx.x * C::N + c + 4096u * (4u * g + i);
                             x = cmul(x, epi_twiddle(epi, pos, sg));
                         }
                         o[j][i] = pack(x);
                     }
                 }
-                const u32 c0 = 2048u * r + 4u * h + 8u * (w & 1) + 16u * c16 + 256u * (w >> 1);
+                const u32 c0 = 2048u * r + 4u * c16 + 64u * h + 128u * w;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     u32 q[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
-                    store4(sbase + c0 + 4096u * (4u * g + i), q);
+                    store4_nt(sbase + c0 + 4096u * (4u * g + i), q);
                 }
             }
         } else {
-            // reader wave w': columns c = j + 4g + 16*c16 + 256*(q + QN*w') (+2048r)
+            // reader wave w': columns c = j + 4*lane + 256*(q + QN*w') (+2048r)
             constexpr int QN = C::COLS / (256 * W);
 #pragma unroll
             for (int q = 0; q < QN; ++q) {
                 u32 v[W][4];
 #pragma unroll
-                for (int s = 0; s < W; ++s) load4(v[s], lds + (u32)s * C::COLS + 4u * lane + 256u * (q + QN * w));
+                for (int s = 0; s < W; ++s)
+                    load4(v[s], lds + (u32)s * C::COLS + lds_sigma(4u * lane + 256u * (q + QN * w)));
                 u32 o[W][4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -533,15 +490,15 @@ __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, int sign,
                     for (int s = 0; s < W; ++s) {
                         cf y = x[s];
                         if constexpr (EPI) {
-                            const u32 c = 2048u * r + j + 4u * g + 16u * c16 + 256u * (q + QN * w);
+                            const u32 c = 2048u * r + j + 4u * lane + 256u * (q + QN * w);
                             y = cmul(y, epi_twiddle(epi, blockIdx.x * C::N + c + 4096u * s, sg));
                         }
                         o[s][j] = pack(y);
                     }
                 }
-                const u32 c0 = 2048u * r + 4u * g + 16u * c16 + 256u * (q + QN * w);
+                const u32 c0 = 2048u * r + 4u * lane + 256u * (q + QN * w);
 #pragma unroll
-                for (int s = 0; s < W; ++s) store4(sbase + c0 + 4096u * s, o[s]);
+                for (int s = 0; s < W; ++s) store4_nt(sbase + c0 + 4096u * s, o[s]);
             }
         }
     }
@@ -549,14 +506,20 @@ __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, int sign,
 
 // ----------------------------------------------------------------------------
 // Host-side launchers (called by the pass driver).
+template <int T, int SIGN>
+static hipError_t launch_rows_t(u32* buf, u32 n_tiles, const float2* w4096, const EpiParams* epi, hipStream_t st) {
+    const u32 waves = (n_tiles + 15) / 16;
+    const u32 blocks = (waves + 3) / 4;
+    if (epi) hipLaunchKernelGGL((k_rows<T, true, SIGN>), dim3(blocks), dim3(256), 0, st, buf, n_tiles, w4096, *epi);
+    else hipLaunchKernelGGL((k_rows<T, false, SIGN>), dim3(blocks), dim3(256), 0, st, buf, n_tiles, w4096, EpiParams{});
+    return hipGetLastError();
+}
+
 template <int T>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
                                 hipStream_t st) {
-    const u32 waves = (n_tiles + 15) / 16;
-    const u32 blocks = (waves + 3) / 4;
-    if (epi) hipLaunchKernelGGL((k_rows<T, true>), dim3(blocks), dim3(256), 0, st, buf, n_tiles, sign, w4096, *epi);
-    else hipLaunchKernelGGL((k_rows<T, false>), dim3(blocks), dim3(256), 0, st, buf, n_tiles, sign, w4096, EpiParams{});
-    return hipGetLastError();
+    return sign < 0 ? launch_rows_t<T, -1>(buf, n_tiles, w4096, epi, st)
+                    : launch_rows_t<T, +1>(buf, n_tiles, w4096, epi, st);
 }
 
 hipError_t launch_rows(int T, u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
@@ -606,23 +569,41 @@ hipError_t launch_cols(int R, u32* buf, u32 s_log2, u32 outer, int sign, const f
     }
 }
 
-template <int W>
-static hipError_t launch_big_t(u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
+// Raise the dynamic-LDS limit of a kernel once per process (thread-safe).
+template <class K>
+static hipError_t set_lds_once(K kernel, size_t bytes) {
+    static std::once_flag flag;
+    static hipError_t status = hipSuccess;
+    std::call_once(flag, [&] {
+        status = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+    });
+    return status;
+}
+
+template <int W, int SIGN>
+static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, const float2* wn_hi,
                                const float2* wn_lo, u32 lo_bits, const EpiParams* epi, hipStream_t st) {
     const size_t lds = BigCfg<W>::LDS_WORDS * sizeof(u32);
     if (epi) {
-        auto k = k_big<W, true>;
-        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        auto k = k_big<W, true, SIGN>;
+        hipError_t e = set_lds_once(k, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, sign, w4096, wn_hi, wn_lo, lo_bits, *epi);
+        hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits, *epi);
     } else {
-        auto k = k_big<W, false>;
-        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        auto k = k_big<W, false, SIGN>;
+        hipError_t e = set_lds_once(k, lds);
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, sign, w4096, wn_hi, wn_lo, lo_bits,
+        hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits,
                            EpiParams{});
     }
     return hipGetLastError();
+}
+
+template <int W>
+static hipError_t launch_big_t(u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
+                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, hipStream_t st) {
+    return sign < 0 ? launch_big_t<W, -1>(buf, n_signals, w4096, wn_hi, wn_lo, lo_bits, epi, st)
+                    : launch_big_t<W, +1>(buf, n_signals, w4096, wn_hi, wn_lo, lo_bits, epi, st);
 }
 
 hipError_t launch_big(int W, u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
