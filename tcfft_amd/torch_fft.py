@@ -1,0 +1,118 @@
+"""Optional PyTorch-ROCm binding over the C ABI: differentiable FP16 FFTs.
+
+Complex fp16 tensors are carried as real fp16 tensors of shape [..., n, 2]
+(interleaved re/im, the kernels' layout).  Transforms run along the n axis,
+in natural order, on the current HIP stream, through libtcfft.so.
+
+    fft(x)           X = F x          (forward, exp(-2*pi*i*nk/n), unnormalised)
+    fft_adjoint(x)   Y = F^H x        (conjugate-twiddle kernel, unnormalised)
+
+Autograd: both are complex-linear maps, and for a complex-linear A viewed as a
+real-linear map on [..., 2] the transpose is the complex adjoint A^H.  So the
+backward of fft is the hand-written adjoint kernel (F^H) and the backward of
+fft_adjoint is the forward kernel (F).  No CPU fallback: these raise if the
+library or a GPU is missing.
+
+FFTConv1d implements y = F^H (H * F x) / n with a learnable filter spectrum H
+broadcast over leading dimensions (circular convolution by the impulse
+response F^H H / n).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import FORWARD, INVERSE, Plan1D
+
+_PLANS: dict = {}
+
+
+def _plan(n: int, batch: int, device: torch.device) -> Plan1D:
+    key = (n, batch, device.index)
+    p = _PLANS.get(key)
+    if p is None:
+        with torch.cuda.device(device):
+            p = Plan1D(n, batch)
+        _PLANS[key] = p
+    return p
+
+
+def _check(x: torch.Tensor):
+    if x.device.type != "cuda":
+        raise RuntimeError("tcfft_amd.torch_fft needs a CUDA (HIP) tensor")
+    if x.dtype != torch.float16 or x.dim() < 2 or x.shape[-1] != 2:
+        raise RuntimeError("expected an fp16 tensor of shape [..., n, 2] (interleaved complex)")
+
+
+def _exec(x: torch.Tensor, direction: int) -> torch.Tensor:
+    _check(x)
+    x = x.contiguous()
+    n = x.shape[-2]
+    batch = x.numel() // (2 * n)
+    out = torch.empty_like(x)
+    plan = _plan(n, batch, x.device)
+    stream = torch.cuda.current_stream(x.device).cuda_stream
+    plan.exec_natural(x.data_ptr(), out.data_ptr(), direction, stream)
+    return out
+
+
+class _FFT(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        return _exec(x, FORWARD)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _exec(g, INVERSE)
+
+
+class _FFTAdjoint(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        return _exec(x, INVERSE)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _exec(g, FORWARD)
+
+
+def fft(x: torch.Tensor) -> torch.Tensor:
+    """Differentiable forward FFT of an fp16 [..., n, 2] tensor."""
+    return _FFT.apply(x)
+
+
+def fft_adjoint(x: torch.Tensor) -> torch.Tensor:
+    """Differentiable unnormalised inverse (adjoint) FFT, F^H x."""
+    return _FFTAdjoint.apply(x)
+
+
+def ifft(x: torch.Tensor) -> torch.Tensor:
+    """Normalised inverse, F^H x / n (the scale is applied in fp32)."""
+    n = x.shape[-2]
+    return (fft_adjoint(x).float() / n).half()
+
+
+def complex_mul(a: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
+    """Pointwise complex product of [..., n, 2] tensors, computed in fp32,
+    returned in a's dtype; h broadcasts over a's leading dimensions."""
+    ar, ai = a[..., 0].float(), a[..., 1].float()
+    hr, hi = h[..., 0].float(), h[..., 1].float()
+    return torch.stack([ar * hr - ai * hi, ar * hi + ai * hr], dim=-1).to(a.dtype)
+
+
+class FFTConv1d(torch.nn.Module):
+    """y = F^H (H * F x) / n along the n axis of an fp16 [..., C, n, 2] input.
+
+    H has shape [C, n, 2] (a complex filter spectrum per channel) and is kept
+    in fp32 as a parameter; it is cast per call.
+    """
+
+    def __init__(self, channels: int, n: int, std: float = 0.02, device=None):
+        super().__init__()
+        self.n = n
+        self.weight = torch.nn.Parameter(torch.randn(channels, n, 2, device=device) * std)
+
+    def forward(self, x: torch.Tensor) -> torch.Tensor:
+        X = fft(x)
+        Y = complex_mul(X, self.weight)
+        y = fft_adjoint(Y)
+        return (y.float() / self.n).half()
