@@ -46,6 +46,10 @@ def parse():
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--workload", default="plain", choices=["plain", "split"],
+                    help="plain: the headline fp16 exec; split: FP32 in/out split-precision exec "
+                         "(reported with its time ratio to the plain exec)")
+    ap.add_argument("--split-chunk", type=int, default=None, help="signals per split pass pair")
     ap.add_argument("--calib", action="store_true",
                     help="also run 5 device-to-device copies of one 64 MiB buffer (counter calibration kernel)")
     return ap.parse_args()
@@ -83,8 +87,79 @@ def load_traffic(n: int, batch: int):
         return None
 
 
+def graph_time(torch, launch, reps):
+    """Mean ms per launch(i, stream) over reps, captured once into a hipGraph
+    and replayed between synchronizations."""
+    g = torch.cuda.CUDAGraph()
+    cs = torch.cuda.Stream()
+    cs.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(g, stream=cs):
+        for i in range(reps):
+            launch(i, cs.cuda_stream)
+    g.replay()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(torch.cuda.current_stream())
+    g.replay()
+    b.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def bench_split(args):
+    """Split precision, N = 2^16, FP32 complex in/out, N(0,1)*10^U(-3,3) per
+    signal; reports the time ratio to the plain fp16 exec of the same shape."""
+    import torch
+    import tcfft_amd
+    n, batch = 1 << 16, args.batch
+    rot = 4  # 4 x (128 MiB in + 128 MiB out) > MALL
+    rng = np.random.default_rng(0)
+    scale = 10.0 ** rng.uniform(-3, 3, (batch, 1))
+    x = ((rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))) * scale).astype(np.complex64)
+    host = torch.from_numpy(x.view(np.float32))
+    ins = [host.cuda() for _ in range(rot)]
+    outs = [torch.empty_like(ins[0]) for _ in range(rot)]
+    plan = tcfft_amd.Plan1D(n, batch)
+    if args.split_chunk is not None:
+        plan.set_option(tcfft_amd.OPT_SPLIT_CHUNK, args.split_chunk)
+    plan.exec_split(ins[0].data_ptr(), outs[0].data_ptr())
+    torch.cuda.synchronize()
+    k = 8
+    got = outs[0][:k].cpu().numpy().view(np.complex64).astype(np.complex128)
+    ref = np.fft.fft(x[:k].astype(np.complex128), axis=-1)
+    errs = [float(np.linalg.norm(got[i] - ref[i]) / np.linalg.norm(ref[i])) for i in range(k)]
+    for i in range(args.warmup):
+        plan.exec_split(ins[i % rot].data_ptr(), outs[i % rot].data_ptr())
+    ms = graph_time(torch, lambda i, st: plan.exec_split(ins[i % rot].data_ptr(), outs[i % rot].data_ptr(),
+                                                         tcfft_amd.FORWARD, st), args.steps)
+    # plain fp16 exec of the same shape, same run
+    pbufs = [torch.empty((batch, n, 2), dtype=torch.float16, device="cuda") for _ in range(10)]
+    for b in pbufs:
+        b.uniform_(-1, 1)
+    ms_plain = graph_time(torch, lambda i, st: plan.exec(pbufs[i % 10].data_ptr(), tcfft_amd.FORWARD, st), args.steps)
+    flops = 5.0 * n * math.log2(n) * batch
+    out = {"metric": "split-precision FFT GFLOP/s (5N·log2N) at N=2^16; rel-L2 vs FP64 per signal",
+           "value": round(flops / (ms * 1e-3) / 1e9, 1), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f16 hi+lo MFMA, FP32 accumulate, FP32 I/O",
+           "data": "synthetic N(0,1)*10^U(-3,3) per signal, complex64",
+           "config": {"workload": f"1D C2C split-precision FFT N={n} batch={batch}, FP32 in/out, natural order",
+                      "n": n, "batch": batch, "split_chunk": args.split_chunk},
+           "rel_l2_max_per_signal": max(errs), "ms_plain_fp16": round(ms_plain, 5),
+           "time_ratio_vs_plain": round(ms / ms_plain, 3),
+           "alg_bytes": 16 * n * batch, "hbm_frac_alg": round(16 * n * batch / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+    plan.destroy()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     args = parse()
+    if args.workload == "split":
+        if args.gpus != 1:
+            sys.exit(2)
+        bench_split(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batch axis shards with "
               f"no exchange, see DESIGN.md. Run with --gpus 1.", file=sys.stderr)

@@ -82,6 +82,19 @@ int tcfft_exec(tcfft_plan plan, void *data, int direction, void *stream);
  * device workspace of the buffer's size, allocated on first use). */
 int tcfft_exec_natural(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
 
+/* Split precision (no counterpart in the reference; config "FP16 hi+lo of
+ * FP32 inputs"): FP32 complex input and output (interleaved float re/im,
+ * natural order, in != out), every radix-16 stage as FP16 hi/lo products on
+ * MFMA with FP32 accumulation, ~1e-7 relative error.  Currently n = 65536
+ * plans only (else TCFFT_INVALID_SIZE). */
+int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
+
+/* Plan options. TCFFT_OPT_SPLIT_CHUNK: signals per pass pair of
+ * tcfft_exec_split (0 = whole batch; default 32), which bounds the
+ * intermediate's footprint so the second pass reads it from cache. */
+enum tcfft_option { TCFFT_OPT_SPLIT_CHUNK = 1 };
+int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
+
 /* Runs passes [first, first + count) of the plan only (the buffer must hold
  * what the earlier passes produced).  For profiling individual pass kernels;
  * tcfft_exec == tcfft_exec_passes(plan, data, dir, 0, tcfft_plan_num_passes, s). */

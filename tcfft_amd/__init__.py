@@ -37,7 +37,9 @@ EXPORTS = (
     "tcfft_version", "tcfft_status_string", "tcfft_last_hip_error", "tcfft_plan_create_1d",
     "tcfft_plan_create_2d", "tcfft_plan_destroy", "tcfft_plan_radices", "tcfft_query_1d", "tcfft_query_2d",
     "tcfft_exec", "tcfft_exec_natural", "tcfft_gen_rev", "tcfft_plan_num_passes", "tcfft_exec_passes",
+    "tcfft_exec_split", "tcfft_plan_set_option",
 )
+OPT_SPLIT_CHUNK = 1
 
 
 class TcfftError(RuntimeError):
@@ -77,6 +79,8 @@ def lib():
         L.tcfft_gen_rev.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, _i32p]
         L.tcfft_plan_num_passes.argtypes = [_vp, _i32p]
         L.tcfft_exec_passes.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
+        L.tcfft_exec_split.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+        L.tcfft_plan_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -152,6 +156,13 @@ class _Plan:
     def exec_natural(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
         _check(lib().tcfft_exec_natural(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)),
                "tcfft_exec_natural")
+
+    def exec_split(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
+        """FP32 complex in/out (natural order, in != out), FP16 hi/lo MFMA split precision."""
+        _check(lib().tcfft_exec_split(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)), "tcfft_exec_split")
+
+    def set_option(self, option: int, value: int):
+        _check(lib().tcfft_plan_set_option(self._h, option, value), "tcfft_plan_set_option")
 
     def destroy(self):
         if self._h.value:

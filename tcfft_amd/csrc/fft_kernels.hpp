@@ -28,6 +28,11 @@ hipError_t launch_rows(int T, uint32_t* buf, uint32_t n_tiles, int sign, const f
 hipError_t launch_big(int W, uint32_t* buf, uint32_t n_signals, int sign, const float2* w4096, const float2* wn_hi,
                       const float2* wn_lo, uint32_t lo_bits, const EpiParams* epi, hipStream_t st);
 
+// Split-precision N = 65536 transform of FP32 complex data (in != out), as
+// two passes per chunk of `chunk` signals (0 = all at once).
+hipError_t launch_split_65536(const float2* in, float2* out, uint32_t n_signals, uint32_t chunk, int sign,
+                              const float2* w4096, const float2* wn_hi, const float2* wn_lo, hipStream_t st);
+
 // Strided pass of radix R (2..256) at element step 2^s_log2 over `outer` blocks.
 hipError_t launch_cols(int R, uint32_t* buf, uint32_t s_log2, uint32_t outer, int sign, const float2* w4096,
                        const EpiParams* epi, hipStream_t st);

@@ -182,6 +182,7 @@ struct tcfft_plan_s {
     void* workspace = nullptr;
     size_t workspace_bytes = 0;
     size_t total = 0;  // elements in the buffer
+    int split_chunk = 32;  // signals per pass pair in tcfft_exec_split
 };
 
 namespace {
@@ -403,6 +404,34 @@ int tcfft_exec(tcfft_plan plan, void* data, int direction, void* stream) {
     if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
     DeviceGuard guard(plan->device);
     return run_passes(plan, (uint32_t*)data, direction, (hipStream_t)stream);
+}
+
+int tcfft_exec_split(tcfft_plan plan, const void* in, void* out, int direction, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!in || !out || in == out || ((uintptr_t)in & 15u) || ((uintptr_t)out & 15u)) return TCFFT_INVALID_POINTER;
+    if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
+    const Layout& L = plan->lay;
+    if (L.dims != 1 || L.n != 65536) return TCFFT_INVALID_SIZE;
+    DeviceGuard guard(plan->device);
+    std::pair<float2*, float2*> t;
+    int rc = table_pair(plan, 16, t);
+    if (rc) return rc;
+    hipError_t e = tcfft::launch_split_65536((const float2*)in, (float2*)out, (uint32_t)L.batch,
+                                             (uint32_t)plan->split_chunk, direction, plan->w4096, t.first, t.second,
+                                             (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    switch (option) {
+    case TCFFT_OPT_SPLIT_CHUNK:
+        if (value < 0) return TCFFT_INVALID_VALUE;
+        plan->split_chunk = value;
+        return TCFFT_SUCCESS;
+    default: return TCFFT_INVALID_VALUE;
+    }
 }
 
 int tcfft_plan_num_passes(tcfft_plan plan, int* n_passes) {
