@@ -95,6 +95,25 @@ int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, 
 enum tcfft_option { TCFFT_OPT_SPLIT_CHUNK = 1 };
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
+/* Uneven batches (no counterpart in the reference): n_signals transforms
+ * of sizes[i] (powers of two, 2^8 .. 2^16) packed back to back in one
+ * buffer (signal i at element offset sizes[0] + ... + sizes[i-1]), each in
+ * the drop-in order (digit-reversed by the radix list a 1D plan of that size
+ * reports), transformed in place by one persistent launch.  Workgroups walk
+ * equal-sized tasks (one size class, <= 65536 elements each) round-robin. */
+typedef struct tcfft_multi_s *tcfft_multi;
+int tcfft_multi_create(tcfft_multi *plan, int n_signals, const int *sizes);
+int tcfft_multi_destroy(tcfft_multi plan);
+int tcfft_multi_info(tcfft_multi plan, int *n_tasks, long long *total_elements);
+int tcfft_multi_exec(tcfft_multi plan, void *data, int direction, void *stream);
+/* Schedule: grid = persistent workgroups (0 = one per task), order = task
+ * order (0 largest sizes first, 1 interleaved, 2 smallest first). */
+int tcfft_multi_set_schedule(tcfft_multi plan, int grid, int order);
+/* Times every (grid, order) candidate on `data` (which it overwrites) and
+ * keeps the fastest. */
+int tcfft_multi_autotune(tcfft_multi plan, void *data, void *stream, int *best_grid, int *best_order,
+                         float *best_ms);
+
 /* Runs passes [first, first + count) of the plan only (the buffer must hold
  * what the earlier passes produced).  For profiling individual pass kernels;
  * tcfft_exec == tcfft_exec_passes(plan, data, dir, 0, tcfft_plan_num_passes, s). */

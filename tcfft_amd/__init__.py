@@ -38,6 +38,8 @@ EXPORTS = (
     "tcfft_plan_create_2d", "tcfft_plan_destroy", "tcfft_plan_radices", "tcfft_query_1d", "tcfft_query_2d",
     "tcfft_exec", "tcfft_exec_natural", "tcfft_gen_rev", "tcfft_plan_num_passes", "tcfft_exec_passes",
     "tcfft_exec_split", "tcfft_plan_set_option",
+    "tcfft_multi_create", "tcfft_multi_destroy", "tcfft_multi_info", "tcfft_multi_exec",
+    "tcfft_multi_set_schedule", "tcfft_multi_autotune",
 )
 OPT_SPLIT_CHUNK = 1
 
@@ -81,6 +83,12 @@ def lib():
         L.tcfft_exec_passes.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         L.tcfft_exec_split.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_plan_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.tcfft_multi_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, _i32p]
+        L.tcfft_multi_destroy.argtypes = [_vp]
+        L.tcfft_multi_info.argtypes = [_vp, _i32p, ctypes.POINTER(ctypes.c_longlong)]
+        L.tcfft_multi_exec.argtypes = [_vp, _vp, ctypes.c_int, _vp]
+        L.tcfft_multi_set_schedule.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.tcfft_multi_autotune.argtypes = [_vp, _vp, _vp, _i32p, _i32p, ctypes.POINTER(ctypes.c_float)]
         _lib = L
     return _lib
 
@@ -219,3 +227,46 @@ class Plan2D(_Plan):
     @property
     def numel(self) -> int:
         return self.nx * self.ny * self.batch
+
+
+class MultiPlan:
+    """Uneven batch: signals of sizes[i] (2^8..2^16) packed back to back,
+    each digit-reversed by its own 1D radix list, one persistent launch."""
+
+    def __init__(self, sizes):
+        self.sizes = np.ascontiguousarray(sizes, dtype=np.int32)
+        self._h = _vp()
+        _check(lib().tcfft_multi_create(ctypes.byref(self._h), len(self.sizes), self.sizes.ctypes.data_as(_i32p)),
+               "tcfft_multi_create")
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes, dtype=np.int64)[:-1]])
+        self.total = int(self.sizes.sum())
+
+    @property
+    def n_tasks(self) -> int:
+        n = ctypes.c_int()
+        _check(lib().tcfft_multi_info(self._h, ctypes.byref(n), None), "tcfft_multi_info")
+        return n.value
+
+    def exec(self, data_ptr: int, direction: int = FORWARD, stream: int = 0):
+        _check(lib().tcfft_multi_exec(self._h, _vp(data_ptr), direction, _vp(stream)), "tcfft_multi_exec")
+
+    def set_schedule(self, grid: int, order: int):
+        _check(lib().tcfft_multi_set_schedule(self._h, grid, order), "tcfft_multi_set_schedule")
+
+    def autotune(self, data_ptr: int, stream: int = 0):
+        g, o, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        _check(lib().tcfft_multi_autotune(self._h, _vp(data_ptr), _vp(stream), ctypes.byref(g), ctypes.byref(o),
+                                          ctypes.byref(ms)), "tcfft_multi_autotune")
+        return g.value, o.value, ms.value
+
+    def destroy(self):
+        if self._h.value:
+            _check(lib().tcfft_multi_destroy(self._h), "tcfft_multi_destroy")
+            self._h = _vp()
+
+    def __del__(self):
+        try:
+            if self._h.value and _lib is not None:
+                _lib.tcfft_multi_destroy(self._h)
+        except Exception:
+            pass

@@ -100,16 +100,15 @@ __device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, con
 // radix list [16, 16] (+ [T] when T > 1).  Position in a line:
 // p = d0 + 16*d1 + 256*d2.  Wave layout at load: tile t = 256-element block,
 // lane (c16 = d1, g) holds d0 = 4g..4g+3 in its four words (one 16-byte load).
-template <int T, bool EPI, int SIGN>
-__global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles,
-                                              const float2* __restrict__ w4096, EpiParams epi) {
+// One wave's work: 16 tiles of 256 elements; tile t lives at tile_ptr(t)
+// (already offset by this lane's 16*c16 + 4*g); tiles t >= ntl are absent.
+// tile0 is the global tile index of tile 0 (used by the epilogue only).
+template <int T, bool EPI, int SIGN, class TilePtr>
+__device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, const float2* __restrict__ w4096,
+                                          const EpiParams& epi) {
     constexpr int sign = SIGN;
     const int lane = threadIdx.x & 63;
     const int c16 = lane & 15, g = lane >> 4;
-    const u32 wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    const u32 tile0 = wave * 16u;
-    if (tile0 >= n_tiles) return;
-    const u32 ntl = min(16u, n_tiles - tile0);
     const float sg = sign < 0 ? 1.f : -1.f;
 
     const Dft16Op op = make_dft16(lane, sign);
@@ -118,10 +117,9 @@ __global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles
     for (int i = 0; i < 4; ++i) tw1[i] = twiddle(w4096, ((c16 * (4 * g + i)) & 255) * 16, sg);
 
     u32 d[16][4];
-    u32* const base = buf + (size_t)tile0 * 256 + 16 * c16 + 4 * g;
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
-        if ((u32)t < ntl) load4_nt(d[t], base + t * 256);
+        if ((u32)t < ntl) load4_nt(d[t], tile_ptr(t));
         else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
     }
 
@@ -153,7 +151,7 @@ __global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles
                 }
                 d[t][i] = pack(v);
             }
-            if ((u32)t < ntl) store4(base + t * 256, d[t]);
+            if ((u32)t < ntl) store4(tile_ptr(t), d[t]);
         }
     } else if constexpr (T < 16) {
         // Stage 1 on each tile of a line, then radix-T across the line's tiles:
@@ -193,7 +191,7 @@ __global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles
                     }
                     d[t][i] = pack(v);
                 }
-                if ((u32)t < ntl) store4(base + t * 256, d[t]);
+                if ((u32)t < ntl) store4(tile_ptr(t), d[t]);
             }
         }
     } else {
@@ -210,8 +208,19 @@ __global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles
         transpose_gi_tile(d, e);
 #pragma unroll
         for (int t = 0; t < 16; ++t)
-            if ((u32)t < ntl) store4_nt(base + t * 256, e[t]);
+            if ((u32)t < ntl) store4_nt(tile_ptr(t), e[t]);
     }
+}
+
+template <int T, bool EPI, int SIGN>
+__global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles,
+                                              const float2* __restrict__ w4096, EpiParams epi) {
+    const int lane = threadIdx.x & 63;
+    const u32 wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const u32 tile0 = wave * 16u;
+    if (tile0 >= n_tiles) return;
+    u32* const base = buf + (size_t)tile0 * 256 + 16 * (lane & 15) + 4 * (lane >> 4);
+    rows_body<T, EPI, SIGN>([&](int t) { return base + t * 256; }, min(16u, n_tiles - tile0), tile0, w4096, epi);
 }
 
 // ----------------------------------------------------------------------------
@@ -378,32 +387,37 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
 // the final global stores are 256-byte runs per 16 lanes.
 __device__ __forceinline__ u32 lds_sigma(u32 c) { return c ^ (((c >> 4) & 15u) << 2); }
 
-template <int W>
+template <int W, int ROUNDS_ = (W >= 8 ? 2 : 1)>
 struct BigCfg {
-    static constexpr int ROUNDS = W >= 8 ? 2 : 1;
+    static constexpr int ROUNDS = ROUNDS_;
     static constexpr int COLS = 4096 / ROUNDS;  // columns exchanged per round
     static constexpr int LDS_WORDS = W * COLS;
     static constexpr u32 N = 4096u * W;
 };
 
-template <int W, bool EPI, int SIGN>
-__global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, const float2* __restrict__ w4096,
-                                                const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
-                                                u32 wn_lo_bits, EpiParams epi) {
+// One signal of N = 4096*W at sbase, processed by the W waves of a group
+// (this wave is number w) with lds = the group's W*COLS words of LDS.
+// Every barrier is a workgroup barrier: all groups of a workgroup run the
+// same sequence.  sig is the signal's index (epilogue positions only).
+template <int W, int ROUNDS, bool EPI, int SIGN>
+__device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* lds, u32 sig,
+                                         const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
+                                         const float2* __restrict__ wn_lo, u32 wn_lo_bits, const EpiParams& epi,
+                                         bool active = true) {
     constexpr int sign = SIGN;
-    using C = BigCfg<W>;
-    extern __shared__ u32 lds[];
+    using C = BigCfg<W, ROUNDS>;
     const int lane = threadIdx.x & 63;
     const int c16 = lane & 15, g = lane >> 4;
-    const int w = threadIdx.x >> 6;
     const float sg = sign < 0 ? 1.f : -1.f;
-    u32* const sbase = buf + (size_t)blockIdx.x * C::N;
 
     u32 d[16][4];
     {
         const u32* src = sbase + 4096 * w + 16 * c16 + 4 * g;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) load4_nt(d[t], src + 256 * t);
+        for (int t = 0; t < 16; ++t) {
+            if (active) load4_nt(d[t], src + 256 * t);
+            else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
+        }
     }
     // W_N^m for m < N from the two-level table
     auto wn = [&](u32 m) -> cf {
@@ -457,7 +471,7 @@ __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, const flo
                         cf x = {re[i], im[i]};
                         if constexpr (EPI) {
                             const u32 c = 2048u * r + j + 4u * c16 + 64u * h + 128u * w;
-                            const u32 pos = blockIdx.x * C::N + c + 4096u * (4u * g + i);
+                            const u32 pos = sig * C::N + c + 4096u * (4u * g + i);
                             x = cmul(x, epi_twiddle(epi, pos, sg));
                         }
                         o[j][i] = pack(x);
@@ -467,7 +481,7 @@ __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, const flo
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     u32 q[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
-                    store4_nt(sbase + c0 + 4096u * (4u * g + i), q);
+                    if (active) store4_nt(sbase + c0 + 4096u * (4u * g + i), q);
                 }
             }
         } else {
@@ -491,17 +505,27 @@ __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, const flo
                         cf y = x[s];
                         if constexpr (EPI) {
                             const u32 c = 2048u * r + j + 4u * lane + 256u * (q + QN * w);
-                            y = cmul(y, epi_twiddle(epi, blockIdx.x * C::N + c + 4096u * s, sg));
+                            y = cmul(y, epi_twiddle(epi, sig * C::N + c + 4096u * s, sg));
                         }
                         o[s][j] = pack(y);
                     }
                 }
                 const u32 c0 = 2048u * r + 4u * lane + 256u * (q + QN * w);
 #pragma unroll
-                for (int s = 0; s < W; ++s) store4_nt(sbase + c0 + 4096u * s, o[s]);
+                for (int s = 0; s < W; ++s)
+                    if (active) store4_nt(sbase + c0 + 4096u * s, o[s]);
             }
         }
     }
+}
+
+template <int W, bool EPI, int SIGN>
+__global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, const float2* __restrict__ w4096,
+                                                const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
+                                                u32 wn_lo_bits, EpiParams epi) {
+    extern __shared__ u32 lds[];
+    big_body<W, BigCfg<W>::ROUNDS, EPI, SIGN>(buf + (size_t)blockIdx.x * BigCfg<W>::N, threadIdx.x >> 6, lds,
+                                             blockIdx.x, w4096, wn_hi, wn_lo, wn_lo_bits, epi);
 }
 
 // ----------------------------------------------------------------------------
@@ -615,6 +639,82 @@ hipError_t launch_big(int W, u32* buf, u32 n_signals, int sign, const float2* w4
     case 16: return launch_big_t<16>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// ----------------------------------------------------------------------------
+// k_multi: many transforms of uneven power-of-two sizes (2^8 .. 2^16) in one
+// persistent launch.  The host cuts the batch into tasks of one size class
+// and at most 65536 elements: 16/T lines of L = 256*T per wave for L <= 4096
+// (rows_body), or 16/W signals of N = 4096*W per workgroup for larger N
+// (big_body, W waves and W*8 KiB of LDS each).  Workgroup b runs tasks
+// b, b + grid, b + 2*grid, ... (no atomics, so the launch is graph-safe).
+template <int T, int SIGN>
+__device__ __forceinline__ void multi_rows(u32* buf, const MultiTask& tk, const u32* __restrict__ offsets,
+                                           const float2* w4096) {
+    constexpr int LPW = 16 / T;  // lines per wave
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int first_line = wave * LPW;
+    const int lines = min((int)tk.count - first_line, LPW);
+    if (lines <= 0) return;
+    const u32 lane_off = 16u * (lane & 15) + 4u * (lane >> 4);
+    const u32* offs = offsets + tk.first + first_line;
+    rows_body<T, false, SIGN>([&](int t) { return buf + offs[t / T] + 256u * (t % T) + lane_off; }, (u32)(lines * T),
+                              0u, w4096, EpiParams{});
+}
+
+template <int W, int SIGN>
+__device__ __forceinline__ void multi_big(u32* buf, const MultiTask& tk, const u32* __restrict__ offsets,
+                                          const float2* w4096, const MultiTables& tb, u32* lds) {
+    constexpr int GROUPS = 16 / W;
+    constexpr int M = W == 2 ? 13 : W == 4 ? 14 : W == 8 ? 15 : 16;
+    const int wave = threadIdx.x >> 6;
+    const int grp = wave / W;
+    const bool active = grp < (int)tk.count;
+    const u32 off = active ? offsets[tk.first + grp] : 0u;
+    big_body<W, 2, false, SIGN>(buf + off, wave % W, lds + grp * (W * 2048), 0u, w4096, tb.hi[M - 13],
+                                tb.lo[M - 13], (u32)(M / 2), EpiParams{}, active);
+}
+
+template <int SIGN>
+__global__ __launch_bounds__(1024) void k_multi(u32* __restrict__ buf, const MultiTask* __restrict__ tasks,
+                                                u32 n_tasks, const u32* __restrict__ offsets,
+                                                const float2* __restrict__ w4096, MultiTables tb) {
+    extern __shared__ u32 lds[];
+    for (u32 ti = blockIdx.x; ti < n_tasks; ti += gridDim.x) {
+        const MultiTask tk = tasks[ti];
+        switch (tk.size_log2) {
+        case 8: multi_rows<1, SIGN>(buf, tk, offsets, w4096); break;
+        case 9: multi_rows<2, SIGN>(buf, tk, offsets, w4096); break;
+        case 10: multi_rows<4, SIGN>(buf, tk, offsets, w4096); break;
+        case 11: multi_rows<8, SIGN>(buf, tk, offsets, w4096); break;
+        case 12: multi_rows<16, SIGN>(buf, tk, offsets, w4096); break;
+        case 13: multi_big<2, SIGN>(buf, tk, offsets, w4096, tb, lds); break;
+        case 14: multi_big<4, SIGN>(buf, tk, offsets, w4096, tb, lds); break;
+        case 15: multi_big<8, SIGN>(buf, tk, offsets, w4096, tb, lds); break;
+        case 16: multi_big<16, SIGN>(buf, tk, offsets, w4096, tb, lds); break;
+        default: break;
+        }
+        __syncthreads();  // the next task reuses the LDS
+    }
+}
+
+template <int SIGN>
+static hipError_t launch_multi_t(u32* buf, const MultiTask* tasks, u32 n_tasks, const u32* offsets, u32 grid,
+                                 const float2* w4096, const MultiTables& tb, hipStream_t st) {
+    constexpr size_t lds = 16 * 2048 * sizeof(u32);  // 128 KiB
+    auto k = k_multi<SIGN>;
+    hipError_t e = set_lds_once(k, lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1024), lds, st, buf, tasks, n_tasks, offsets, w4096, tb);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi(u32* buf, const MultiTask* tasks, u32 n_tasks, const u32* offsets, u32 grid, int sign,
+                        const float2* w4096, const MultiTables& tb, hipStream_t st) {
+    if (n_tasks == 0) return hipSuccess;
+    if (grid == 0 || grid > n_tasks) grid = n_tasks;
+    return sign < 0 ? launch_multi_t<-1>(buf, tasks, n_tasks, offsets, grid, w4096, tb, st)
+                    : launch_multi_t<+1>(buf, tasks, n_tasks, offsets, grid, w4096, tb, st);
 }
 
 }  // namespace tcfft

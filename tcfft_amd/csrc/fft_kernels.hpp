@@ -33,6 +33,20 @@ hipError_t launch_big(int W, uint32_t* buf, uint32_t n_signals, int sign, const 
 hipError_t launch_split_65536(const float2* in, float2* out, uint32_t n_signals, uint32_t chunk, int sign,
                               const float2* w4096, const float2* wn_hi, const float2* wn_lo, hipStream_t st);
 
+// Uneven batch: one task = signals [first, first + count) of one size class.
+struct MultiTask {
+    uint32_t size_log2;
+    uint32_t first;
+    uint32_t count;
+    uint32_t pad;
+};
+struct MultiTables {  // two-level W_N tables for N = 2^13 .. 2^16
+    const float2* hi[4];
+    const float2* lo[4];
+};
+hipError_t launch_multi(uint32_t* buf, const MultiTask* tasks, uint32_t n_tasks, const uint32_t* offsets,
+                        uint32_t grid, int sign, const float2* w4096, const MultiTables& tb, hipStream_t st);
+
 // Strided pass of radix R (2..256) at element step 2^s_log2 over `outer` blocks.
 hipError_t launch_cols(int R, uint32_t* buf, uint32_t s_log2, uint32_t outer, int sign, const float2* w4096,
                        const EpiParams* epi, hipStream_t st);

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <new>
 #include <vector>
@@ -505,6 +506,246 @@ int tcfft_gen_rev(int n, const int* radices, int n_radices, int* rev) {
         rev[p] = idx;
     }
     return TCFFT_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Uneven batches (no counterpart in the reference): many signals of sizes
+// 2^8 .. 2^16 packed back to back, transformed by one persistent launch.
+namespace {
+constexpr uint32_t kMultiMagic = 0x7cff3a17u;
+}
+
+struct tcfft_multi_s {
+    uint32_t magic = kMultiMagic;
+    int device = 0;
+    std::vector<int> sizes;
+    std::vector<uint32_t> offsets;  // element offset of every signal
+    size_t total = 0;
+    std::vector<tcfft::MultiTask> tasks;
+    std::vector<uint32_t> task_offsets;  // offsets in task order
+    tcfft::MultiTask* d_tasks = nullptr;
+    uint32_t* d_offsets = nullptr;
+    float2* w4096 = nullptr;
+    tcfft::MultiTables tb{};
+    uint32_t grid = 256;
+    int order = 0;
+    std::vector<void*> allocs;
+};
+
+namespace {
+
+bool valid_multi(tcfft_multi_s* p) { return p && p->magic == kMultiMagic; }
+
+int multi_alloc(tcfft_multi_s* p, size_t bytes, void** out) {
+    hipError_t e = hipMalloc(out, bytes);
+    if (e != hipSuccess) return hip_fail(e);
+    p->allocs.push_back(*out);
+    return TCFFT_SUCCESS;
+}
+
+// Build the task list for an ordering policy:
+//   0: size classes from the largest to the smallest (longest tasks first)
+//   1: classes interleaved task by task
+//   2: size classes from the smallest to the largest
+void multi_build_tasks(tcfft_multi_s* p, int order) {
+    std::vector<std::vector<uint32_t>> byclass(17);
+    for (size_t i = 0; i < p->sizes.size(); ++i) byclass[ilog2((uint32_t)p->sizes[i])].push_back((uint32_t)i);
+    std::vector<std::vector<tcfft::MultiTask>> cls_tasks(17);
+    std::vector<std::vector<std::vector<uint32_t>>> cls_offs(17);
+    for (int m = 8; m <= 16; ++m) {
+        const auto& ids = byclass[m];
+        const int per_task = m <= 12 ? 256 / (1 << (m - 8)) : 16 / (1 << (m - 12));
+        for (size_t s0 = 0; s0 < ids.size(); s0 += per_task) {
+            const size_t cnt = std::min(ids.size() - s0, (size_t)per_task);
+            std::vector<uint32_t> offs;
+            for (size_t k = 0; k < cnt; ++k) offs.push_back(p->offsets[ids[s0 + k]]);
+            cls_tasks[m].push_back(tcfft::MultiTask{(uint32_t)m, 0u, (uint32_t)cnt, 0u});
+            cls_offs[m].push_back(offs);
+        }
+    }
+    p->tasks.clear();
+    p->task_offsets.clear();
+    auto emit = [&](int m, size_t k) {
+        tcfft::MultiTask t = cls_tasks[m][k];
+        t.first = (uint32_t)p->task_offsets.size();
+        for (uint32_t o : cls_offs[m][k]) p->task_offsets.push_back(o);
+        p->tasks.push_back(t);
+    };
+    if (order == 1) {
+        size_t most = 0;
+        for (int m = 8; m <= 16; ++m) most = std::max(most, cls_tasks[m].size());
+        for (size_t k = 0; k < most; ++k)
+            for (int m = 16; m >= 8; --m)
+                if (k < cls_tasks[m].size()) emit(m, k);
+    } else {
+        for (int i = 0; i < 9; ++i) {
+            const int m = order == 2 ? 8 + i : 16 - i;
+            for (size_t k = 0; k < cls_tasks[m].size(); ++k) emit(m, k);
+        }
+    }
+}
+
+int multi_upload_tasks(tcfft_multi_s* p) {
+    hipError_t e = hipMemcpy(p->d_tasks, p->tasks.data(), p->tasks.size() * sizeof(tcfft::MultiTask),
+                             hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e);
+    e = hipMemcpy(p->d_offsets, p->task_offsets.data(), p->task_offsets.size() * sizeof(uint32_t),
+                  hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
+void multi_free(tcfft_multi_s* p) {
+    for (void* a : p->allocs) (void)hipFree(a);
+    p->magic = 0;
+    delete p;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tcfft_multi_create(tcfft_multi* plan, int n_signals, const int* sizes) {
+    if (!plan || !sizes) return TCFFT_INVALID_POINTER;
+    *plan = nullptr;
+    if (n_signals < 1) return TCFFT_INVALID_BATCH;
+    size_t total = 0;
+    for (int i = 0; i < n_signals; ++i) {
+        if (!is_pow2(sizes[i]) || sizes[i] < 256 || sizes[i] > 65536) return TCFFT_INVALID_SIZE;
+        total += (size_t)sizes[i];
+    }
+    if (total >= (1ull << 31)) return TCFFT_INVALID_BATCH;
+    tcfft_multi_s* p = new (std::nothrow) tcfft_multi_s;
+    if (!p) return TCFFT_ALLOC_FAILED;
+    p->sizes.assign(sizes, sizes + n_signals);
+    p->offsets.resize(n_signals);
+    size_t acc = 0;
+    for (int i = 0; i < n_signals; ++i) {
+        p->offsets[i] = (uint32_t)acc;
+        acc += (size_t)sizes[i];
+    }
+    p->total = total;
+    int rc = TCFFT_SUCCESS;
+    hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) rc = hip_fail(e);
+    multi_build_tasks(p, 0);
+    void* d = nullptr;
+    if (!rc) rc = multi_alloc(p, p->tasks.size() * sizeof(tcfft::MultiTask), &d);
+    p->d_tasks = (tcfft::MultiTask*)d;
+    if (!rc) rc = multi_alloc(p, std::max<size_t>(p->task_offsets.size(), 1) * sizeof(uint32_t), &d);
+    p->d_offsets = (uint32_t*)d;
+    if (!rc) rc = multi_upload_tasks(p);
+    // tables: W_4096 and two-level W_N for N = 2^13 .. 2^16
+    auto up = [&](const std::vector<float2>& h, const float2** dst) {
+        if (rc) return;
+        void* dd = nullptr;
+        rc = multi_alloc(p, h.size() * sizeof(float2), &dd);
+        if (rc) return;
+        hipError_t ee = hipMemcpy(dd, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice);
+        if (ee != hipSuccess) rc = hip_fail(ee);
+        *dst = (const float2*)dd;
+    };
+    const float2* w = nullptr;
+    up(host_table(12, 1, 4096), &w);
+    p->w4096 = (float2*)w;
+    for (int m = 13; m <= 16; ++m) {
+        const uint32_t lo = (uint32_t)m / 2;
+        up(host_table((uint32_t)m, 1ull << lo, 1u << (m - lo)), &p->tb.hi[m - 13]);
+        up(host_table((uint32_t)m, 1, 1u << lo), &p->tb.lo[m - 13]);
+    }
+    if (rc) {
+        multi_free(p);
+        return rc;
+    }
+    *plan = p;
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_multi_destroy(tcfft_multi plan) {
+    if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
+    DeviceGuard guard(plan->device);
+    multi_free(plan);
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_multi_info(tcfft_multi plan, int* n_tasks, long long* total_elements) {
+    if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
+    if (n_tasks) *n_tasks = (int)plan->tasks.size();
+    if (total_elements) *total_elements = (long long)plan->total;
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_multi_set_schedule(tcfft_multi plan, int grid, int order) {
+    if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
+    if (grid < 0 || order < 0 || order > 2) return TCFFT_INVALID_VALUE;
+    DeviceGuard guard(plan->device);
+    plan->grid = (uint32_t)grid;
+    if (order != plan->order) {
+        plan->order = order;
+        multi_build_tasks(plan, order);
+        return multi_upload_tasks(plan);
+    }
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_multi_exec(tcfft_multi plan, void* data, int direction, void* stream) {
+    if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
+    if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
+    if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
+    DeviceGuard guard(plan->device);
+    hipError_t e = tcfft::launch_multi((uint32_t*)data, plan->d_tasks, (uint32_t)plan->tasks.size(), plan->d_offsets,
+                                       plan->grid, direction, plan->w4096, plan->tb, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_multi_autotune(tcfft_multi plan, void* data, void* stream, int* best_grid, int* best_order,
+                         float* best_ms) {
+    if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
+    if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
+    DeviceGuard guard(plan->device);
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return TCFFT_HIP_ERROR;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return TCFFT_HIP_ERROR;
+    }
+    int cu = 256;
+    (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, plan->device);
+    const int grids[] = {cu / 2, cu, 2 * cu};
+    float best = 1e30f;
+    int bg = cu, bo = 0, rc = TCFFT_SUCCESS;
+    for (int order = 0; order < 3 && !rc; ++order) {
+        for (int gsz : grids) {
+            rc = tcfft_multi_set_schedule(plan, gsz, order);
+            if (rc) break;
+            for (int i = 0; i < 2 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
+            if (rc) break;
+            (void)hipEventRecord(a, st);
+            for (int i = 0; i < 5 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
+            (void)hipEventRecord(b, st);
+            if (hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, a, b);
+            ms /= 5.f;
+            if (!rc && ms < best) {
+                best = ms;
+                bg = gsz;
+                bo = order;
+            }
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
+    rc = tcfft_multi_set_schedule(plan, bg, bo);
+    if (best_grid) *best_grid = bg;
+    if (best_order) *best_order = bo;
+    if (best_ms) *best_ms = best;
+    return rc;
 }
 
 }  // extern "C"
