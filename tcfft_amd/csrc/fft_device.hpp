@@ -74,6 +74,16 @@ struct Dft16Op {
     f16x8 re, im;
 };
 
+// Lane index within the wave.  The value passes through an empty volatile
+// asm so that the compiler cannot hoist lane-derived constants (DFT-16
+// operands, twiddles) out of a persistent task loop, where they would stay
+// live across every task and exceed the register budget.
+__device__ __forceinline__ int lane_id() {
+    int l = (int)(threadIdx.x & 63u);
+    asm volatile("" : "+v"(l));
+    return l;
+}
+
 // Constant DFT-16 operand for this lane.  sign = -1 forward, +1 adjoint.
 __device__ __forceinline__ Dft16Op make_dft16(int lane, int sign) {
     const int f = lane & 15, g = lane >> 4;

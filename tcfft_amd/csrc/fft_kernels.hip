@@ -100,14 +100,16 @@ __device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, con
 // radix list [16, 16] (+ [T] when T > 1).  Position in a line:
 // p = d0 + 16*d1 + 256*d2.  Wave layout at load: tile t = 256-element block,
 // lane (c16 = d1, g) holds d0 = 4g..4g+3 in its four words (one 16-byte load).
-// One wave's work: 16 tiles of 256 elements; tile t lives at tile_ptr(t)
-// (already offset by this lane's 16*c16 + 4*g); tiles t >= ntl are absent.
-// tile0 is the global tile index of tile 0 (used by the epilogue only).
-template <int T, bool EPI, int SIGN, class TilePtr>
+// One call's work: NT tiles of 256 elements (NT = 16, or a multiple of T
+// below 16 to bound register use); tile t lives at tile_ptr(t) (already
+// offset by this lane's 16*c16 + 4*g); tiles t >= ntl are absent.  tile0 is
+// the global tile index of tile 0 (used by the epilogue only).
+template <int T, bool EPI, int SIGN, int NT = 16, class TilePtr>
 __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, const float2* __restrict__ w4096,
                                           const EpiParams& epi) {
+    static_assert(NT % T == 0 && NT <= 16 && (T < 16 || NT == 16), "tile count per call");
     constexpr int sign = SIGN;
-    const int lane = threadIdx.x & 63;
+    const int lane = lane_id();
     const int c16 = lane & 15, g = lane >> 4;
     const float sg = sign < 0 ? 1.f : -1.f;
 
@@ -116,9 +118,9 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
 #pragma unroll
     for (int i = 0; i < 4; ++i) tw1[i] = twiddle(w4096, ((c16 * (4 * g + i)) & 255) * 16, sg);
 
-    u32 d[16][4];
+    u32 d[NT][4];
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
+    for (int t = 0; t < NT; ++t) {
         if ((u32)t < ntl) load4_nt(d[t], tile_ptr(t));
         else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
     }
@@ -127,7 +129,7 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
     // lane%16 = k0, rows (g,i) = d1.  Twiddle W_256^{k0*d1} is a per-lane constant.
     if constexpr (T < 16) {
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
+        for (int t = 0; t < NT; ++t) {
             f32x4 re, im;
             dft16_a(d[t], op, re, im);
 #pragma unroll
@@ -139,7 +141,7 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
         // Stage 1: DFT over d1 -> lane%16 = k1, rows (g,i) = k0; natural
         // position c = k0 + 16*k1 = 4g + i + 16*c16: four adjacent words.
 #pragma unroll
-        for (int t = 0; t < 16; ++t) {
+        for (int t = 0; t < NT; ++t) {
             f32x4 re, im;
             dft16_a(d[t], op, re, im);
 #pragma unroll
@@ -158,7 +160,7 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
         // X[c + 256*k2] = sum_d2 W_{256T}^{c*d2} Y_d2[c] W_T^{d2*k2}.
         constexpr int STRIDE = 16 / T;  // W_{256T}^m = W_4096^{m*STRIDE}
 #pragma unroll
-        for (int gi = 0; gi < 16 / T; ++gi) {
+        for (int gi = 0; gi < NT / T; ++gi) {
             cf acc[T][4];
 #pragma unroll
             for (int tt = 0; tt < T; ++tt) {
@@ -406,7 +408,7 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
                                          bool active = true) {
     constexpr int sign = SIGN;
     using C = BigCfg<W, ROUNDS>;
-    const int lane = threadIdx.x & 63;
+    const int lane = lane_id();
     const int c16 = lane & 15, g = lane >> 4;
     const float sg = sign < 0 ? 1.f : -1.f;
 
@@ -415,8 +417,7 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
         const u32* src = sbase + 4096 * w + 16 * c16 + 4 * g;
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
-            if (active) load4_nt(d[t], src + 256 * t);
-            else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
+            load4_nt(d[t], src + 256 * t);
         }
     }
     // W_N^m for m < N from the two-level table
@@ -652,14 +653,28 @@ template <int T, int SIGN>
 __device__ __forceinline__ void multi_rows(u32* buf, const MultiTask& tk, const u32* __restrict__ offsets,
                                            const float2* w4096) {
     constexpr int LPW = 16 / T;  // lines per wave
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    // wave-uniform (scalar) so that the per-line offsets are scalar loads
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int first_line = wave * LPW;
     const int lines = min((int)tk.count - first_line, LPW);
     if (lines <= 0) return;
     const u32 lane_off = 16u * (lane & 15) + 4u * (lane >> 4);
     const u32* offs = offsets + tk.first + first_line;
-    rows_body<T, false, SIGN>([&](int t) { return buf + offs[t / T] + 256u * (t % T) + lane_off; }, (u32)(lines * T),
-                              0u, w4096, EpiParams{});
+    if constexpr (T == 16) {
+        rows_body<T, false, SIGN>([&](int t) { return buf + offs[t / T] + 256u * (t % T) + lane_off; },
+                                  (u32)(lines * T), 0u, w4096, EpiParams{});
+    } else {
+        // calls of NT tiles keep the line data within the register budget of
+        // a 1024-thread workgroup (128 VGPRs); NT per T is the largest that
+        // compiles without spills
+        constexpr int NT = (T == 1 || T == 8) ? 8 : 4;
+        for (int t0 = 0; t0 < lines * T; t0 += NT) {
+            const int ntl = lines * T - t0;
+            rows_body<T, false, SIGN, NT>(
+                [&](int t) { return buf + offs[(t0 + t) / T] + 256u * ((t0 + t) % T) + lane_off; }, (u32)ntl, 0u,
+                w4096, EpiParams{});
+        }
+    }
 }
 
 template <int W, int SIGN>
@@ -667,10 +682,12 @@ __device__ __forceinline__ void multi_big(u32* buf, const MultiTask& tk, const u
                                           const float2* w4096, const MultiTables& tb, u32* lds) {
     constexpr int GROUPS = 16 / W;
     constexpr int M = W == 2 ? 13 : W == 4 ? 14 : W == 8 ? 15 : 16;
-    const int wave = threadIdx.x >> 6;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int grp = wave / W;
+    // A group without a signal (last task of a class) loads the task's first
+    // signal, so that every load stays in bounds, and stores nothing.
     const bool active = grp < (int)tk.count;
-    const u32 off = active ? offsets[tk.first + grp] : 0u;
+    const u32 off = offsets[tk.first + (active ? grp : 0)];
     big_body<W, 2, false, SIGN>(buf + off, wave % W, lds + grp * (W * 2048), 0u, w4096, tb.hi[M - 13],
                                 tb.lo[M - 13], (u32)(M / 2), EpiParams{}, active);
 }
