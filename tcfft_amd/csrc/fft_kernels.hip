@@ -406,6 +406,7 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
                                          const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
                                          const float2* __restrict__ wn_lo, u32 wn_lo_bits, const EpiParams& epi,
                                          bool active = true) {
+    static_assert(!EPI || W == 16, "the epilogue twiddle is folded for W = 16 only");
     constexpr int sign = SIGN;
     using C = BigCfg<W, ROUNDS>;
     const int lane = lane_id();
@@ -420,17 +421,36 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
             load4_nt(d[t], src + 256 * t);
         }
     }
-    // W_N^m for m < N from the two-level table
-    auto wn = [&](u32 m) -> cf {
-        m &= C::N - 1u;
-        return cmul(twiddle(wn_hi, m >> wn_lo_bits, sg), twiddle(wn_lo, m & ((1u << wn_lo_bits) - 1u), sg));
+    // Phase-1 twiddle W_N^{w*c} = W_Q^{c*e} with root W_Q and exponent e:
+    //   no epilogue: Q = N, e = w;
+    //   epilogue (W = 16, next pass of radix R = 2^r at step N over chunks
+    //   j = sig mod R): the next pass's twiddle W_{NR}^{(c + 4096k)*rev(j)}
+    //   splits into W_{NR}^{c*rev(j)}, merged here as Q = N*R,
+    //   e = w*R + rev(j), and W_{NR}^{4096*k*rev(j)}, a row scale of the
+    //   final-stage operand (output frequency k).
+    u32 e_w = (u32)w, rj = 0u;
+    if constexpr (EPI) {
+        const u32 j = sig & ((1u << epi.r_log2) - 1u);
+        const u32 j0 = j & ((1u << epi.ra_log2) - 1u);
+        rj = (j >> epi.ra_log2) | (j0 << (epi.r_log2 - epi.ra_log2));
+        e_w = ((u32)w << epi.r_log2) + rj;
+    }
+    // W_Q^m from the two-level table of Q
+    auto wq = [&](u32 m) -> cf {
+        if constexpr (EPI) {
+            m &= epi.m_mask;
+            return cmul(twiddle(epi.hi, m >> epi.lo_bits, sg), twiddle(epi.lo, m & ((1u << epi.lo_bits) - 1u), sg));
+        } else {
+            m &= C::N - 1u;
+            return cmul(twiddle(wn_hi, m >> wn_lo_bits, sg), twiddle(wn_lo, m & ((1u << wn_lo_bits) - 1u), sg));
+        }
     };
     const Dft16Op op = make_dft16(lane, sign);
-    // stage-2 operand rows f = lane%16 scaled by W_N^{256*w*f}
-    const Dft16Op op2 = make_dft16_scaled(lane, sign, wn(256u * (u32)w * (u32)c16));
-    // per-(lane, tile) factor W_N^{w*(16*k1 + k0)}, k1 = lane%16, k0 = tile
-    cf fac = wn(16u * (u32)w * (u32)c16);
-    const cf fstep = wn((u32)w);
+    // stage-2 operand rows f = lane%16 scaled by W_Q^{256*e*f}
+    const Dft16Op op2 = make_dft16_scaled(lane, sign, wq(256u * e_w * (u32)c16));
+    // per-(lane, tile) factor W_Q^{e*(16*k1 + k0)}, k1 = lane%16, k0 = tile
+    cf fac = wq(16u * e_w * (u32)c16);
+    const cf fstep = wq(e_w);
     dit4096(d, lane, sign, w4096, op, op2, [&](int t, int i, cf v) -> cf {
         if (i == 0 && t > 0) fac = cmul(fac, fstep);
         return cmul(v, fac);
@@ -438,6 +458,10 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
     // d: tile = k0, lane%16 = k1, (g,i) = k2.  Writer layout: word = c bits 0-1
     u32 e[16][4];
     transpose_gi_tile(d, e);  // (g,i) = k0, tile = k2: c = (4g+i) + 16*c16 + 256*t
+    // final-stage operand (W = 16): rows k scaled by W_Q^{4096*k*rev(j)} under
+    // the epilogue; built after the first exchange barrier (a fresh lane id)
+    // so that it does not hold registers during the in-wave DIT
+    Dft16Op op3 = op;
 
 #pragma unroll
     for (int r = 0; r < C::ROUNDS; ++r) {
@@ -449,6 +473,12 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
             store4(lds + off, e[r * TPR + tt]);
         }
         __syncthreads();
+        if constexpr (EPI) {
+            if (r == 0) {
+                const int ln = lane_id();
+                op3 = make_dft16_scaled(ln, sign, wq(4096u * rj * (u32)(ln & 15)));
+            }
+        }
         if constexpr (W == 16) {
             // reader wave w': columns c = j + 4*c16 + 64h + 128w' (+2048r)
             u32 v[2][4][4];  // [h][i = point (w = 4g+i)][j]
@@ -466,17 +496,9 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
                 for (int j = 0; j < 4; ++j) {
                     u32 a[4] = {v[h][0][j], v[h][1][j], v[h][2][j], v[h][3][j]};
                     f32x4 re, im;
-                    dft16_b(a, op, re, im);
+                    dft16_b(a, op3, re, im);
 #pragma unroll
-                    for (int i = 0; i < 4; ++i) {
-                        cf x = {re[i], im[i]};
-                        if constexpr (EPI) {
-                            const u32 c = 2048u * r + j + 4u * c16 + 64u * h + 128u * w;
-                            const u32 pos = sig * C::N + c + 4096u * (4u * g + i);
-                            x = cmul(x, epi_twiddle(epi, pos, sg));
-                        }
-                        o[j][i] = pack(x);
-                    }
+                    for (int i = 0; i < 4; ++i) o[j][i] = pack(cf{re[i], im[i]});
                 }
                 const u32 c0 = 2048u * r + 4u * c16 + 64u * h + 128u * w;
 #pragma unroll
@@ -502,14 +524,7 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
                     for (int s = 0; s < W; ++s) x[s] = unpack(v[s][j]);
                     small_dft<W>(x, sign);
 #pragma unroll
-                    for (int s = 0; s < W; ++s) {
-                        cf y = x[s];
-                        if constexpr (EPI) {
-                            const u32 c = 2048u * r + j + 4u * lane + 256u * (q + QN * w);
-                            y = cmul(y, epi_twiddle(epi, sig * C::N + c + 4096u * s, sg));
-                        }
-                        o[s][j] = pack(y);
-                    }
+                    for (int s = 0; s < W; ++s) o[s][j] = pack(x[s]);
                 }
                 const u32 c0 = 2048u * r + 4u * lane + 256u * (q + QN * w);
 #pragma unroll
@@ -610,10 +625,17 @@ static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, con
                                const float2* wn_lo, u32 lo_bits, const EpiParams* epi, hipStream_t st) {
     const size_t lds = BigCfg<W>::LDS_WORDS * sizeof(u32);
     if (epi) {
-        auto k = k_big<W, true, SIGN>;
-        hipError_t e = set_lds_once(k, lds);
-        if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits, *epi);
+        // the folded epilogue covers a first pass of 65536 contiguous points
+        // followed by a pass at step 65536 along the same axis
+        if constexpr (W == 16) {
+            if (epi->shift != 0 || epi->s_log2 != 16) return hipErrorInvalidValue;
+            auto k = k_big<W, true, SIGN>;
+            hipError_t e = set_lds_once(k, lds);
+            if (e != hipSuccess) return e;
+            hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits, *epi);
+        } else {
+            return hipErrorInvalidValue;
+        }
     } else {
         auto k = k_big<W, false, SIGN>;
         hipError_t e = set_lds_once(k, lds);

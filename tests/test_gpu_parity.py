@@ -104,6 +104,21 @@ def test_multipass_forward_vs_oracle(n, batch):
     assert err <= tol_for(n)
 
 
+@pytest.mark.parametrize("n,batch,direction", [(1 << 19, 1, tcfft_amd.FORWARD), (1 << 20, 2, tcfft_amd.FORWARD),
+                                               (1 << 20, 1, tcfft_amd.INVERSE), (1 << 22, 1, tcfft_amd.FORWARD),
+                                               (1 << 24, 1, tcfft_amd.FORWARD), (1 << 25, 1, tcfft_amd.INVERSE)])
+def test_large_vs_numpy(n, batch, direction):
+    """Sizes above 2^18 (every radix of the pass after the 65536-point
+    first pass) against numpy's FP64 FFT; the oracle is pinned to numpy at
+    smaller sizes by the CPU tests."""
+    x = fp16_input((batch, n), seed=n + 11)
+    got = run_1d(x, direction)
+    ref = np.fft.fft(x, axis=-1) if direction == tcfft_amd.FORWARD else np.fft.ifft(x, axis=-1) * n
+    err = oracle.rel_l2(got, ref)
+    note(f"  rel_l2={err:.3e}")
+    assert err <= tol_for(n)
+
+
 @pytest.mark.parametrize("n,batch", [(256, 4), (1024, 2), (4096, 3), (65536, 1), (1 << 17, 1)])
 def test_inverse_vs_oracle(n, batch):
     x = fp16_input((batch, n), seed=3 * n)
