@@ -40,6 +40,23 @@ __device__ __forceinline__ cf epi_twiddle(const EpiParams& e, u32 gpos, float sg
     return cmul(a, b);
 }
 
+// Strided passes see their output at (block blk, row, column col) of a
+// buffer viewed as [blk][R][s]; along the axis that is P = blk*R*sa + row*sa
+// + xc with sa = s >> shift and xc = col >> shift, so the next pass's twiddle
+// is W_M^{(row*sa + xc)*rev(blk mod R')}: a per-column factor times a
+// per-row factor, both powers of one root.  epi_rev gives rev(blk mod R'),
+// epi_w gives W_M^m from the two-level table.
+__device__ __forceinline__ u32 epi_rev(const EpiParams& e, u32 blk) {
+    const u32 j = blk & ((1u << e.r_log2) - 1u);
+    const u32 j0 = j & ((1u << e.ra_log2) - 1u);
+    return (j >> e.ra_log2) | (j0 << (e.r_log2 - e.ra_log2));
+}
+
+__device__ __forceinline__ cf epi_w(const EpiParams& e, u32 m, float sgn) {
+    m &= e.m_mask;
+    return cmul(twiddle(e.hi, m >> e.lo_bits, sgn), twiddle(e.lo, m & ((1u << e.lo_bits) - 1u), sgn));
+}
+
 // ----------------------------------------------------------------------------
 // In-wave 4096-point DIT over the digits d0, d1, d2 of a 4096-element chunk.
 // In : d[t][j] = tile t (d2), lane (c16 = d1, g), word j (d0 = 4g + j).
@@ -47,10 +64,14 @@ __device__ __forceinline__ cf epi_twiddle(const EpiParams& e, u32 gpos, float sg
 //      is applied to every FP32 result of the last stage.
 // Stage-1 twiddle W_256^{k0*d1} is a per-lane constant; the stage-2 twiddle
 // W_4096^{(k0+16k1)*d2} is a per-lane recurrence over the tiles.  The
-// last stage uses the operand op2 (B-form), which may carry a row scale.
-template <class Post>
+// last stage uses the operand make_op2() (B-form), which may carry a row
+// scale; it is built after stage 1 so that it holds no registers before.
+// With REC0 the stage-2 inputs of column c = k0 + 16*k1 are also multiplied
+// by rec0[i] (k0 = 4g + i, k1 = lane%16): a per-column factor of the caller
+// merged into the stage-1 recurrence (rec = rec0 * W_4096^{c*d2}).
+template <bool REC0, class MakeOp2, class Post>
 __device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, const float2* __restrict__ w4096,
-                                        const Dft16Op& op, const Dft16Op& op2, Post post) {
+                                        const Dft16Op& op, MakeOp2 make_op2, const cf (&rec0)[4], Post post) {
     const int c16 = lane & 15, g = lane >> 4;
     const float sg = sign < 0 ? 1.f : -1.f;
     cf tw1[4], step[4], rec[4];
@@ -58,7 +79,7 @@ __device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, con
     for (int i = 0; i < 4; ++i) {
         tw1[i] = twiddle(w4096, ((c16 * (4 * g + i)) & 255) * 16, sg);
         step[i] = twiddle(w4096, 16 * c16 + 4 * g + i, sg);
-        rec[i] = step[i];
+        rec[i] = REC0 ? rec0[i] : step[i];
     }
     // stage 0: DFT over d0 -> lane%16 = k0, (g,i) = d1
 #pragma unroll
@@ -76,15 +97,16 @@ __device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, con
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             cf v = {re[i], im[i]};
-            if (t) {
+            if (REC0 || t) {
                 v = cmul(v, rec[i]);
-                rec[i] = cmul(rec[i], step[i]);
+                if (t < 15) rec[i] = cmul(rec[i], step[i]);
             }
             d[t][i] = pack(v);
         }
     }
     u32 e[16][4];
     transpose_gi_tile(d, e);  // (g,i) = d2, tile = k0
+    const Dft16Op op2 = make_op2();
     // stage 2 (B-form): columns k1, points d2 -> (g,i) = k2
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
@@ -198,7 +220,8 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
         }
     } else {
         // T == 16: one 4096-point line per wave (shared in-wave DIT).
-        dit4096(d, lane, sign, w4096, op, op, [&](int t, int i, cf v) -> cf {
+        const cf none[4] = {};
+        dit4096<false>(d, lane, sign, w4096, op, [&] { return op; }, none, [&](int t, int i, cf v) -> cf {
             if constexpr (EPI) {
                 const u32 k = (u32)t + 16u * c16 + 256u * (4u * g + i);
                 v = cmul(v, epi_twiddle(epi, tile0 * 256u + k, sg));
@@ -253,7 +276,18 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
 #pragma unroll
         for (int i = 0; i < 4; ++i) d[t][i] = buf[blk_base + (size_t)(16 * t + 4 * g + i) * s];
 
-    auto out_pos = [&](u32 row) -> u32 { return (u32)(blk_base + (size_t)row * s); };
+    // Epilogue: output row k0 + 16*t (T < 16), t + 16*k0 (T = 16) or k0
+    // (T = 1), k0 = 4g + i, gets F0[i] * S^t with F0[i] = W_M^{(xc + k0*sa*u)*rj}
+    // (u = 16 for T = 16, else 1) and S = W_M^{sa*rj*(T == 16 ? 1 : 16)}.
+    cf F0[4], S = {1.f, 0.f};
+    if constexpr (EPI) {
+        const u32 rj = epi_rev(epi, blk);
+        const u32 sa = s >> epi.shift, xc = (col0 + (u32)c16) >> epi.shift;
+        const u32 u = T == 16 ? 16u : 1u;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) F0[i] = epi_w(epi, (xc + (4u * g + i) * u * sa) * rj, sg);
+        if constexpr (T > 1) S = epi_w(epi, sa * rj * (T == 16 ? 1u : 16u), sg);
+    }
 
     if constexpr (T == 1) {
         f32x4 re, im;
@@ -262,7 +296,7 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
         for (int i = 0; i < 4; ++i) {
             cf v = {re[i], im[i]};
             const u32 row = 4u * g + i;
-            if constexpr (EPI) v = cmul(v, epi_twiddle(epi, out_pos(row), sg));
+            if constexpr (EPI) v = cmul(v, F0[i]);
             buf[blk_base + (size_t)row * s] = pack(v);
         }
     } else if constexpr (T < 16) {
@@ -283,11 +317,15 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
 #pragma unroll
             for (int t = 1; t < T; ++t) v[t] = cmul(acc[t][i], twiddle(w4096, ((k0 * t) * STRIDE) & 4095u, sg));
             small_dft<T>(v, sign);
+            cf f = EPI ? F0[i] : cf{1.f, 0.f};
 #pragma unroll
             for (int t = 0; t < T; ++t) {
                 const u32 row = k0 + 16u * t;
                 cf o = v[t];
-                if constexpr (EPI) o = cmul(o, epi_twiddle(epi, out_pos(row), sg));
+                if constexpr (EPI) {
+                    o = cmul(o, f);
+                    f = cmul(f, S);
+                }
                 buf[blk_base + (size_t)row * s] = pack(o);
             }
         }
@@ -330,7 +368,10 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
             for (int i = 0; i < 4; ++i) {
                 cf v = {re[i], im[i]};
                 const u32 row = (u32)t + 16u * (4u * g + i);
-                if constexpr (EPI) v = cmul(v, epi_twiddle(epi, out_pos(row), sg));
+                if constexpr (EPI) {
+                    v = cmul(v, F0[i]);
+                    F0[i] = cmul(F0[i], S);
+                }
                 buf[blk_base + (size_t)row * s] = pack(v);
             }
         }
@@ -354,6 +395,16 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
     u32 w[R][4];
 #pragma unroll
     for (int j = 0; j < R; ++j) load4(w[j], buf + base + (size_t)j * s);
+    // Epilogue: output (row j, column col + q) gets W_M^{(j*sa + xc_q)*rj};
+    // xc_q = xc + q when shift = 0, else xc (4 adjacent columns, one x).
+    cf F[R], Q = {1.f, 0.f};
+    if constexpr (EPI) {
+        const u32 rj = epi_rev(epi, blk);
+        const u32 sa = s >> epi.shift, xc = col >> epi.shift;
+#pragma unroll
+        for (int j = 0; j < R; ++j) F[j] = epi_w(epi, (xc + (u32)j * sa) * rj, sg);
+        if (epi.shift == 0) Q = epi_w(epi, rj, sg);
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         cf v[R];
@@ -363,7 +414,10 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             cf o = v[j];
-            if constexpr (EPI) o = cmul(o, epi_twiddle(epi, (u32)(base + (size_t)j * s + q), sg));
+            if constexpr (EPI) {
+                o = cmul(o, F[j]);
+                F[j] = cmul(F[j], Q);
+            }
             w[j][q] = pack(o);
         }
     }
@@ -446,15 +500,17 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
         }
     };
     const Dft16Op op = make_dft16(lane, sign);
-    // stage-2 operand rows f = lane%16 scaled by W_Q^{256*e*f}
-    const Dft16Op op2 = make_dft16_scaled(lane, sign, wq(256u * e_w * (u32)c16));
-    // per-(lane, tile) factor W_Q^{e*(16*k1 + k0)}, k1 = lane%16, k0 = tile
-    cf fac = wq(16u * e_w * (u32)c16);
-    const cf fstep = wq(e_w);
-    dit4096(d, lane, sign, w4096, op, op2, [&](int t, int i, cf v) -> cf {
-        if (i == 0 && t > 0) fac = cmul(fac, fstep);
-        return cmul(v, fac);
-    });
+    // c = k0 + 16*k1 + 256*k2: W_Q^{e*256*k2} scales the stage-2 operand rows
+    // f = k2; W_Q^{e*(k0 + 16*k1)} is constant over k2, so it multiplies the
+    // stage-2 inputs of column (k0, k1), merged into the stage-1 recurrence.
+    cf rec0[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) rec0[i] = wq(e_w * (4u * g + i + 16u * c16));
+    auto make_op2 = [&] {
+        const int ln = lane_id();  // keeps the table loads after stage 1
+        return make_dft16_scaled(ln, sign, wq(256u * e_w * (u32)(ln & 15)));
+    };
+    dit4096<true>(d, lane, sign, w4096, op, make_op2, rec0, [](int, int, cf v) -> cf { return v; });
     // d: tile = k0, lane%16 = k1, (g,i) = k2.  Writer layout: word = c bits 0-1
     u32 e[16][4];
     transpose_gi_tile(d, e);  // (g,i) = k0, tile = k2: c = (4g+i) + 16*c16 + 256*t
