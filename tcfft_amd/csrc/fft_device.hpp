@@ -60,15 +60,29 @@ __device__ __forceinline__ cf twiddle(const float2* __restrict__ tab, u32 idx, f
     return {t.x, sgn_im * t.y};
 }
 
-// cos(2*pi*m/16), m = 0..15, to FP32 precision (used for the DFT-16 operand).
+// cos(2*pi*m/16), m = 0..15 (used for the DFT-16 operand), from the
+// hardware cosine (argument in revolutions, m/16 exact) rather than a table:
+// a per-lane table lookup is a global load, and global loads complete in
+// issue order, so it would wait behind the data loads.  The results are
+// within a few FP32 ulps, far from every FP16 rounding boundary of these
+// values (the nearest is 0.029 FP16 ulp away), so the FP16 operand is the
+// correctly rounded DFT-16 matrix; |cos| below 2^-20 is the exact zero.
 __device__ __forceinline__ float cos16(int m) {
+    const float c = __builtin_amdgcn_cosf((float)(m & 15) * 0.0625f);
+    return fabsf(c) < 0x1p-20f ? 0.f : c;
+}
+__device__ __forceinline__ float sin16(int m) { return cos16(m - 4); }
+
+// Correctly rounded FP32 cos(2*pi*m/16) (split precision keeps the FP32
+// value's low part, so the hardware cosine's last-ulp error would show).
+__device__ __forceinline__ float cos16_exact(int m) {
     constexpr float c[16] = {1.0f, 0.92387953251128674f, 0.70710678118654752f, 0.38268343236508977f,
                              0.0f, -0.38268343236508977f, -0.70710678118654752f, -0.92387953251128674f,
                              -1.0f, -0.92387953251128674f, -0.70710678118654752f, -0.38268343236508977f,
                              0.0f, 0.38268343236508977f, 0.70710678118654752f, 0.92387953251128674f};
     return c[m & 15];
 }
-__device__ __forceinline__ float sin16(int m) { return cos16(m - 4); }
+__device__ __forceinline__ float sin16_exact(int m) { return cos16_exact(m - 4); }
 
 struct Dft16Op {
     f16x8 re, im;

@@ -69,16 +69,31 @@ __device__ __forceinline__ cf epi_w(const EpiParams& e, u32 m, float sgn) {
 // With REC0 the stage-2 inputs of column c = k0 + 16*k1 are also multiplied
 // by rec0[i] (k0 = 4g + i, k1 = lane%16): a per-column factor of the caller
 // merged into the stage-1 recurrence (rec = rec0 * W_4096^{c*d2}).
-template <bool REC0, class MakeOp2, class Post>
-__device__ __forceinline__ void dit4096(u32 (&d)[16][4], int lane, int sign, const float2* __restrict__ w4096,
-                                        const Dft16Op& op, MakeOp2 make_op2, const cf (&rec0)[4], Post post) {
+// The per-lane twiddles come from the caller (Dit4096Tw), loaded before the
+// data: global loads complete in issue order, so a table load issued after
+// the data loads would hold stage 0 until the whole chunk has arrived.
+struct Dit4096Tw {
+    float2 tw1[4], step[4];  // raw table entries: W_256^{k0*d1}, W_4096^{k0+16*k1}
+};
+
+__device__ __forceinline__ void dit4096_tw_load(Dit4096Tw& tw, int lane, const float2* __restrict__ w4096) {
     const int c16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tw.tw1[i] = w4096[((c16 * (4 * g + i)) & 255) * 16];
+        tw.step[i] = w4096[16 * c16 + 4 * g + i];
+    }
+}
+
+template <bool REC0, class MakeOp2, class Post>
+__device__ __forceinline__ void dit4096(u32 (&d)[16][4], int sign, const Dit4096Tw& tw, const Dft16Op& op,
+                                        MakeOp2 make_op2, const cf (&rec0)[4], Post post) {
     const float sg = sign < 0 ? 1.f : -1.f;
     cf tw1[4], step[4], rec[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-        tw1[i] = twiddle(w4096, ((c16 * (4 * g + i)) & 255) * 16, sg);
-        step[i] = twiddle(w4096, 16 * c16 + 4 * g + i, sg);
+        tw1[i] = cf{tw.tw1[i].x, sg * tw.tw1[i].y};
+        step[i] = cf{tw.step[i].x, sg * tw.step[i].y};
         rec[i] = REC0 ? rec0[i] : step[i];
     }
     // stage 0: DFT over d0 -> lane%16 = k0, (g,i) = d1
@@ -135,17 +150,21 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
     const int c16 = lane & 15, g = lane >> 4;
     const float sg = sign < 0 ? 1.f : -1.f;
 
-    const Dft16Op op = make_dft16(lane, sign);
-    cf tw1[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) tw1[i] = twiddle(w4096, ((c16 * (4 * g + i)) & 255) * 16, sg);
-
+    // table loads first, then the data (see Dit4096Tw)
+    Dit4096Tw tw;
+    dit4096_tw_load(tw, lane, w4096);
+    __builtin_amdgcn_sched_barrier(0);
     u32 d[NT][4];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
         if ((u32)t < ntl) load4_nt(d[t], tile_ptr(t));
         else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
     }
+    __builtin_amdgcn_sched_barrier(0);
+    const Dft16Op op = make_dft16(lane, sign);
+    cf tw1[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) tw1[i] = cf{tw.tw1[i].x, sg * tw.tw1[i].y};
 
     // Stage 0 (T < 16; dit4096 does its own): DFT over d0 (A-form) ->
     // lane%16 = k0, rows (g,i) = d1.  Twiddle W_256^{k0*d1} is a per-lane constant.
@@ -221,7 +240,7 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
     } else {
         // T == 16: one 4096-point line per wave (shared in-wave DIT).
         const cf none[4] = {};
-        dit4096<false>(d, lane, sign, w4096, op, [&] { return op; }, none, [&](int t, int i, cf v) -> cf {
+        dit4096<false>(d, sign, tw, op, [&] { return op; }, none, [&](int t, int i, cf v) -> cf {
             if constexpr (EPI) {
                 const u32 k = (u32)t + 16u * c16 + 256u * (4u * g + i);
                 v = cmul(v, epi_twiddle(epi, tile0 * 256u + k, sg));
@@ -467,14 +486,6 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
     const int c16 = lane & 15, g = lane >> 4;
     const float sg = sign < 0 ? 1.f : -1.f;
 
-    u32 d[16][4];
-    {
-        const u32* src = sbase + 4096 * w + 16 * c16 + 4 * g;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-            load4_nt(d[t], src + 256 * t);
-        }
-    }
     // Phase-1 twiddle W_N^{w*c} = W_Q^{c*e} with root W_Q and exponent e:
     //   no epilogue: Q = N, e = w;
     //   epilogue (W = 16, next pass of radix R = 2^r at step N over chunks
@@ -489,28 +500,50 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
         rj = (j >> epi.ra_log2) | (j0 << (epi.r_log2 - epi.ra_log2));
         e_w = ((u32)w << epi.r_log2) + rj;
     }
-    // W_Q^m from the two-level table of Q
-    auto wq = [&](u32 m) -> cf {
-        if constexpr (EPI) {
-            m &= epi.m_mask;
-            return cmul(twiddle(epi.hi, m >> epi.lo_bits, sg), twiddle(epi.lo, m & ((1u << epi.lo_bits) - 1u), sg));
-        } else {
-            m &= C::N - 1u;
-            return cmul(twiddle(wn_hi, m >> wn_lo_bits, sg), twiddle(wn_lo, m & ((1u << wn_lo_bits) - 1u), sg));
-        }
+    // W_Q^m = hi[m >> lo_bits] * lo[m & (2^lo_bits - 1)] (two-level table of Q)
+    const float2* q_hi = EPI ? epi.hi : wn_hi;
+    const float2* q_lo = EPI ? epi.lo : wn_lo;
+    const u32 q_bits = EPI ? epi.lo_bits : wn_lo_bits;
+    const u32 q_mask = EPI ? epi.m_mask : C::N - 1u;
+    struct RawQ {
+        float2 h, l;
     };
+    auto wq_load = [&](u32 m) -> RawQ {
+        m &= q_mask;
+        return RawQ{q_hi[m >> q_bits], q_lo[m & ((1u << q_bits) - 1u)]};
+    };
+    auto wq_val = [&](const RawQ& r) -> cf { return cmul(cf{r.h.x, sg * r.h.y}, cf{r.l.x, sg * r.l.y}); };
+
+    // Table loads first, then the data: global loads complete in issue
+    // order, so table values loaded after the data would hold the first
+    // stage until the whole signal has arrived.
+    //   rec0: W_Q^{e*(k0 + 16*k1)} (k0 = 4g + i, k1 = lane%16), constant over
+    //         k2 for c = k0 + 16*k1 + 256*k2, merged into the stage-1 recurrence;
+    //   s2:   W_Q^{e*256*k2} as a row scale of the stage-2 operand (f = k2);
+    //   s3:   the final-stage row scale under the epilogue.
+    Dit4096Tw tw;
+    dit4096_tw_load(tw, lane, w4096);
+    RawQ r_rec0[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) r_rec0[i] = wq_load(e_w * (4u * g + i + 16u * c16));
+    const RawQ r_s2 = wq_load(256u * e_w * (u32)c16);
+    const RawQ r_s3 = wq_load(4096u * rj * (u32)c16);
+    __builtin_amdgcn_sched_barrier(0);
+    u32 d[16][4];
+    {
+        const u32* src = sbase + 4096 * w + 16 * c16 + 4 * g;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) load4_nt(d[t], src + 256 * t);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+
     const Dft16Op op = make_dft16(lane, sign);
-    // c = k0 + 16*k1 + 256*k2: W_Q^{e*256*k2} scales the stage-2 operand rows
-    // f = k2; W_Q^{e*(k0 + 16*k1)} is constant over k2, so it multiplies the
-    // stage-2 inputs of column (k0, k1), merged into the stage-1 recurrence.
     cf rec0[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) rec0[i] = wq(e_w * (4u * g + i + 16u * c16));
-    auto make_op2 = [&] {
-        const int ln = lane_id();  // keeps the table loads after stage 1
-        return make_dft16_scaled(ln, sign, wq(256u * e_w * (u32)(ln & 15)));
-    };
-    dit4096<true>(d, lane, sign, w4096, op, make_op2, rec0, [](int, int, cf v) -> cf { return v; });
+    for (int i = 0; i < 4; ++i) rec0[i] = wq_val(r_rec0[i]);
+    const cf s2 = wq_val(r_s2);
+    auto make_op2 = [&] { return make_dft16_scaled(lane_id(), sign, s2); };
+    dit4096<true>(d, sign, tw, op, make_op2, rec0, [](int, int, cf v) -> cf { return v; });
     // d: tile = k0, lane%16 = k1, (g,i) = k2.  Writer layout: word = c bits 0-1
     u32 e[16][4];
     transpose_gi_tile(d, e);  // (g,i) = k0, tile = k2: c = (4g+i) + 16*c16 + 256*t
@@ -530,10 +563,7 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
         }
         __syncthreads();
         if constexpr (EPI) {
-            if (r == 0) {
-                const int ln = lane_id();
-                op3 = make_dft16_scaled(ln, sign, wq(4096u * rj * (u32)(ln & 15)));
-            }
+            if (r == 0) op3 = make_dft16_scaled(lane_id(), sign, wq_val(r_s3));
         }
         if constexpr (W == 16) {
             // reader wave w': columns c = j + 4*c16 + 64h + 128w' (+2048r)

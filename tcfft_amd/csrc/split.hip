@@ -34,8 +34,8 @@ __device__ __forceinline__ Dft16Split make_dft16_split(int lane, int sign) {
     for (int j = 0; j < 4; ++j) {
         const int n = 4 * g + j;
         const int m = (n * f) & 15;
-        const float fr = cos16(m);
-        const float fi = sign < 0 ? -sin16(m) : sin16(m);
+        const float fr = cos16_exact(m);
+        const float fi = sign < 0 ? -sin16_exact(m) : sin16_exact(m);
         const _Float16 frh = (_Float16)fr, fih = (_Float16)fi;
         const _Float16 frl = (_Float16)(fr - (float)frh), fil = (_Float16)(fi - (float)fih);
         s.h.re[2 * j] = frh;
