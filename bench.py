@@ -1,21 +1,27 @@
 #!/usr/bin/env python3
-"""Benchmark: fp16 batched 1D FFT, N = 2^16, batch 256 (the headline workload).
+"""Benchmark: fp16 batched 1D FFT, N = 2^16, batch 256 (the headline workload),
+plus the other measured workloads in the same run.
 
 One step = one tcfftExec over the whole batch (256 signals of 65536 complex
 fp16 values, in place, digit-reversed input, natural output) on one MI355X.
 Inputs are synthetic uniform[-1,1) fp16 and already resident in HBM; the timed
-steps rotate over ROTATE buffers (> 512 MiB in total) so no step starts with
-its data in the 256 MB MALL.
+steps rotate over buffers totalling > 640 MiB so no step starts with its data
+in the 256 MB MALL.  Steps are captured once into a hipGraph and replayed.
 
 Prints one JSON line:
   value      = 5*N*log2(N)*batch / step time, GFLOP/s
-  roofline   = dominant pass kernel: algorithmic bytes per launch
+  roofline   = the 65536-point pass kernel: algorithmic bytes per launch
                (8 B per complex element: one fp16 read + one fp16 write)
                / its mean launch time (HIP events on the launch stream),
-               against 8 TB/s; traffic from rocprofv3 counters when a
-               profile summary is committed, else null
+               against 8 TB/s; traffic = HBM bytes per launch from rocprofv3
+               FETCH_SIZE / WRITE_SIZE counters when that summary is committed
   cpu_baseline = the FP64 checker (oracle/, a scalar C restatement of the
                reference algorithm) on a bounded sample, 1 host core
+  cpu_fft_f32  = scipy.fft complex64 on all host cores, bounded sample
+               (the single-precision host FFT the GPU path replaces)
+  configs    = the other workloads (N=4096 x 1024; N=2^20 x 16; 2D 1024^2 x 8;
+               split precision; uneven batch; FFT-conv layer), each with its
+               time, GFLOP/s, HBM fraction and rel-L2 vs FP64
 """
 from __future__ import annotations
 
@@ -33,7 +39,7 @@ sys.path.insert(0, ROOT)
 
 N = 1 << 16
 BATCH = 256
-ROTATE = 10          # 10 x 64 MiB = 640 MiB
+ROTATE_BYTES = 640 << 20   # rotation set > MALL (256 MB)
 PEAK_HBM_GBS = 8000.0
 METRIC = "fp16 FFT GFLOP/s (5N·log2N) + % HBM roofline at N=2^16; rel-L2 vs FP64"
 
@@ -45,15 +51,84 @@ def parse():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=N)
     ap.add_argument("--batch", type=int, default=BATCH)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", default="plain", choices=["plain", "split", "uneven"],
-                    help="plain: the headline fp16 exec; split: FP32 in/out split-precision exec "
-                         "(reported with its time ratio to the plain exec)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
+    ap.add_argument("--no-configs", action="store_true", help="headline workload only")
+    ap.add_argument("--workload", default="plain", choices=["plain", "split", "uneven", "conv"],
+                    help="plain: the headline fp16 exec (+ every config); split / uneven / conv: that leg alone")
     ap.add_argument("--split-chunk", type=int, default=None, help="signals per split pass pair")
     ap.add_argument("--calib", action="store_true",
                     help="also run 5 device-to-device copies of one 64 MiB buffer (counter calibration kernel)")
     return ap.parse_args()
 
+
+# ----------------------------------------------------------------------------
+# timing helpers
+
+def graph_time(torch, launch, reps):
+    """Mean ms per launch(i, stream) over reps, captured once into a hipGraph
+    and replayed between synchronizations (replay() runs on the current
+    stream, so the events are recorded there)."""
+    g = torch.cuda.CUDAGraph()
+    cs = torch.cuda.Stream()
+    cs.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(g, stream=cs):
+        for i in range(reps):
+            launch(i, cs.cuda_stream)
+    g.replay()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record(torch.cuda.current_stream())
+    g.replay()
+    b.record(torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def eager_time(torch, fn, reps):
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record()
+    for i in range(reps):
+        fn(i)
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def rotation(torch, shape, dtype, init):
+    """Buffers of `shape` whose total exceeds ROTATE_BYTES, each = init."""
+    one = int(np.prod(shape)) * torch.tensor([], dtype=dtype).element_size()
+    k = max(2, -(-ROTATE_BYTES // one))
+    bufs = [torch.empty(shape, dtype=dtype, device="cuda") for _ in range(k)]
+    for b in bufs:
+        b.copy_(init)
+    return bufs
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def to_c(t):
+    v = t.float().cpu().numpy().astype(np.float64)
+    return v[..., 0] + 1j * v[..., 1]
+
+
+def fp16_uniform(rng, shape):
+    return rng.uniform(-1, 1, tuple(shape) + (2,)).astype(np.float16)
+
+
+def summary(ms, n_points, batch, alg_bytes):
+    flops = 5.0 * n_points * math.log2(n_points) * batch
+    return {"ms": round(ms, 5), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
+            "hbm_gbs_alg": round(alg_bytes / (ms * 1e-3) / 1e9, 1),
+            "hbm_frac": round(alg_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+
+
+# ----------------------------------------------------------------------------
+# CPU baselines
 
 def cpu_baseline(n: int):
     """FP64 checker on a bounded sample of the same workload (1 core)."""
@@ -75,8 +150,209 @@ def cpu_baseline(n: int):
             "sample": f"{reps} x {sig} signals of N={n} (FP64 DIT restatement, {el:.1f} s)"}
 
 
+def cpu_fft_f32(n: int, batch: int):
+    """scipy.fft complex64 on all host cores, bounded sample of the workload."""
+    import scipy.fft
+    cores = os.cpu_count() or 1
+    rng = np.random.default_rng(0)
+    sig = min(batch, 64)
+    x = (rng.uniform(-1, 1, (sig, n)) + 1j * rng.uniform(-1, 1, (sig, n))).astype(np.complex64)
+    scipy.fft.fft(x, axis=-1, workers=cores)  # warm-up (plan, threads)
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        scipy.fft.fft(x, axis=-1, workers=cores, overwrite_x=False)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > 5.0 or reps >= 200:
+            break
+    flops = 5.0 * n * math.log2(n) * sig * reps
+    gf = flops / el / 1e9
+    return {"value": round(gf, 2), "unit": "GFLOP/s", "cores": cores, "library": f"scipy.fft {scipy.__version__} "
+            "(pocketfft, complex64)", "sample": f"{reps} x {sig} signals of N={n} ({el:.1f} s)",
+            "ms_full_batch_est": round(5.0 * n * math.log2(n) * batch / (gf * 1e9) * 1e3, 3)}
+
+
+# ----------------------------------------------------------------------------
+# workloads
+
+def bench_plan1d(torch, tcfft_amd, n, batch, reps, rng):
+    plan = tcfft_amd.Plan1D(n, batch)
+    x = fp16_uniform(rng, (batch, n))
+    xr = np.ascontiguousarray(x[:, tcfft_amd.gen_rev(n, plan.radices)])
+    bufs = rotation(torch, (batch, n, 2), torch.float16, torch.from_numpy(xr))
+    plan.exec(bufs[0].data_ptr())
+    torch.cuda.synchronize()
+    k = min(batch, 4)
+    err = rel(to_c(bufs[0][:k]), np.fft.fft(x[:k, :, 0].astype(np.float64) + 1j * x[:k, :, 1], axis=-1))
+    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps)
+    out = summary(ms, n, batch, 8.0 * n * batch)
+    out.update({"rel_l2_vs_fp64": err, "passes": plan.num_passes, "radices": plan.radices})
+    plan.destroy()
+    return out
+
+
+def bench_plan2d(torch, tcfft_amd, nx, ny, batch, reps, rng):
+    plan = tcfft_amd.Plan2D(nx, ny, batch)
+    x = fp16_uniform(rng, (batch, nx, ny))
+    rx = tcfft_amd.gen_rev(nx, plan.radices_x)
+    ry = tcfft_amd.gen_rev(ny, plan.radices_y)
+    xr = np.ascontiguousarray(x[:, rx][:, :, ry])
+    bufs = rotation(torch, (batch, nx, ny, 2), torch.float16, torch.from_numpy(xr))
+    plan.exec(bufs[0].data_ptr())
+    torch.cuda.synchronize()
+    err = rel(to_c(bufs[0][:1]), np.fft.fft2(x[:1, :, :, 0].astype(np.float64) + 1j * x[:1, :, :, 1]))
+    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps)
+    out = summary(ms, nx * ny, batch, 8.0 * nx * ny * batch)
+    out.update({"rel_l2_vs_fp64": err, "passes": plan.num_passes, "radices_x": plan.radices_x,
+                "radices_y": plan.radices_y})
+    plan.destroy()
+    return out
+
+
+def bench_split(torch, tcfft_amd, reps, chunk=None, ms_plain=None):
+    """Split precision, N = 2^16 x 256, FP32 complex in/out,
+    N(0,1)*10^U(-3,3) per signal; time ratio to the plain fp16 exec."""
+    n, batch = 1 << 16, 256
+    rng = np.random.default_rng(0)
+    scale = 10.0 ** rng.uniform(-3, 3, (batch, 1))
+    x = ((rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))) * scale).astype(np.complex64)
+    host = torch.from_numpy(x.view(np.float32))
+    ins = rotation(torch, tuple(host.shape), torch.float32, host)
+    outs = [torch.empty_like(ins[0]) for _ in ins]
+    plan = tcfft_amd.Plan1D(n, batch)
+    if chunk is not None:
+        plan.set_option(tcfft_amd.OPT_SPLIT_CHUNK, chunk)
+    plan.exec_split(ins[0].data_ptr(), outs[0].data_ptr())
+    torch.cuda.synchronize()
+    k = 8
+    got = outs[0][:k].cpu().numpy().view(np.complex64).astype(np.complex128)
+    ref = np.fft.fft(x[:k].astype(np.complex128), axis=-1)
+    errs = [rel(got[i], ref[i]) for i in range(k)]
+    ms = graph_time(torch, lambda i, st: plan.exec_split(ins[i % len(ins)].data_ptr(), outs[i % len(ins)].data_ptr(),
+                                                         tcfft_amd.FORWARD, st), reps)
+    if ms_plain is None:
+        pb = rotation(torch, (batch, n, 2), torch.float16, torch.zeros(batch, n, 2, dtype=torch.float16))
+        ms_plain = graph_time(torch, lambda i, st: plan.exec(pb[i % len(pb)].data_ptr(), tcfft_amd.FORWARD, st), reps)
+    plan.destroy()
+    out = summary(ms, n, batch, 16.0 * n * batch)
+    out.update({"rel_l2_max_per_signal": max(errs), "ms_plain_fp16": round(ms_plain, 5),
+                "time_ratio_vs_plain": round(ms / ms_plain, 3), "split_chunk": chunk})
+    return out
+
+
+def bench_uneven(torch, tcfft_amd, reps):
+    """4096 fp16 signals, N = 2^U{8..16} (seed 0), drop-in order: one
+    persistent autotuned launch vs one launch per size class (regular plans
+    on a size-sorted layout of the same volume)."""
+    rng = np.random.default_rng(0)
+    n_sig = 4096
+    sizes = (2 ** rng.integers(8, 17, n_sig)).astype(np.int32)
+    total = int(sizes.sum())
+    flat = rng.uniform(-1, 1, (total, 2)).astype(np.float16)
+    mp = tcfft_amd.MultiPlan(sizes)
+    host = torch.from_numpy(flat)
+    bufs = rotation(torch, tuple(host.shape), torch.float16, host)
+    g, o, _ = mp.autotune(bufs[0].data_ptr())
+    for b in bufs:
+        b.copy_(host)
+    ms_multi = graph_time(torch, lambda i, st: mp.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps)
+    classes = sorted(set(int(v) for v in sizes))
+    plans, offs, o2 = [], [], 0
+    for n in classes:
+        cnt = int((sizes == n).sum())
+        plans.append(tcfft_amd.Plan1D(n, cnt))
+        offs.append(o2)
+        o2 += n * cnt
+
+    def per_size(i, st):
+        base = bufs[i % len(bufs)].data_ptr()
+        for p, off in zip(plans, offs):
+            p.exec(base + 4 * off, tcfft_amd.FORWARD, st)
+
+    ms_sizes = graph_time(torch, per_size, reps)
+    # accuracy on a sample of signals, digit-reversed per signal
+    sample = [int(i) for i in rng.choice(n_sig, 16, replace=False)]
+    check = torch.from_numpy(flat).cuda()
+    for i in sample:
+        n, o3 = int(sizes[i]), int(mp.offsets[i])
+        rad, _ = tcfft_amd.query_1d(n, 1)
+        check[o3:o3 + n] = torch.from_numpy(np.ascontiguousarray(flat[o3:o3 + n][tcfft_amd.gen_rev(n, rad)])).cuda()
+    mp.exec(check.data_ptr())
+    torch.cuda.synchronize()
+    errs = []
+    for i in sample:
+        n, o3 = int(sizes[i]), int(mp.offsets[i])
+        x = flat[o3:o3 + n].astype(np.float64)
+        errs.append(rel(to_c(check[o3:o3 + n]), np.fft.fft(x[:, 0] + 1j * x[:, 1])))
+    flops = float(sum(5.0 * n * math.log2(n) for n in sizes))
+    n_tasks = mp.n_tasks
+    for p in plans:
+        p.destroy()
+    mp.destroy()
+    return {"ms": round(ms_multi, 5), "gflops": round(flops / (ms_multi * 1e-3) / 1e9, 1),
+            "hbm_frac": round(8.0 * total / (ms_multi * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+            "ms_per_size_launches": round(ms_sizes, 5), "speedup_vs_per_size": round(ms_sizes / ms_multi, 3),
+            "signals": n_sig, "elements": total, "tasks": n_tasks, "autotuned_grid": g, "autotuned_order": o,
+            "rel_l2_max_sample": max(errs)}
+
+
+def bench_conv(torch, reps):
+    """FFT-conv layer y = F^H (H * F x) / n, x [64, 256, 8192] real-as-complex
+    fp16, filter spectrum [256, 8192] complex ~ N(0, 0.02): forward and
+    forward + backward per step (eager, torch autograd over the HIP FFTs)."""
+    from tcfft_amd import torch_fft
+    torch.manual_seed(0)
+    B, C, n = 64, 256, 8192
+    layer = torch_fft.FFTConv1d(C, n, std=0.02, device="cuda")
+    xr = (torch.rand(B, C, n, device="cuda") * 2 - 1).half()
+    x = torch.stack([xr, torch.zeros_like(xr)], dim=-1).requires_grad_(True)
+    with torch.no_grad():
+        y = layer(x)
+    torch.cuda.synchronize()
+    hc = layer.weight.detach().double().cpu().numpy()
+    hc = hc[..., 0] + 1j * hc[..., 1]
+    errs = []
+    for (b, c) in ((0, 0), (63, 255)):
+        xs = x[b, c].detach().double().cpu().numpy()
+        ref = np.fft.ifft(np.fft.fft(xs[:, 0] + 1j * xs[:, 1]) * hc[c])
+        errs.append(rel(to_c(y[b, c]), ref))
+
+    def fwd(i):
+        with torch.no_grad():
+            layer(x)
+
+    g = torch.randn(B, C, n, 2, device="cuda").half()
+
+    def fwd_bwd(i):
+        x.grad = None
+        layer.weight.grad = None
+        layer(x).backward(g)
+
+    fwd(0)
+    fwd_bwd(0)
+    ms_f = eager_time(torch, fwd, reps)
+    ms_fb = eager_time(torch, fwd_bwd, max(2, reps // 2))
+    sig = B * C
+    fft_flops = 5.0 * n * math.log2(n) * sig
+    return {"ms_forward": round(ms_f, 4), "ms_forward_backward": round(ms_fb, 4),
+            "fft_gflops_forward": round(2 * fft_flops / (ms_f * 1e-3) / 1e9, 1),
+            "signals": sig, "n": n, "rel_l2_forward_sample": max(errs),
+            "note": "forward = 2 FFTs + pointwise product; forward+backward = 6 FFTs + products (torch autograd)"}
+
+
+def stream_copy(torch):
+    """Device-to-device copy bandwidth over a > MALL rotation (TB/s, read+write)."""
+    src = rotation(torch, (64 << 20,), torch.float32, torch.zeros(64 << 20))  # 256 MiB each
+    dst = [torch.empty_like(src[0]) for _ in range(2)]
+    ms = eager_time(torch, lambda i: dst[i % 2].copy_(src[i % len(src)]), 20)
+    return round(2 * src[0].numel() * 4 / (ms * 1e-3) / 1e12, 3)
+
+
+# ----------------------------------------------------------------------------
+
 def load_traffic(n: int, batch: int):
-    """Counter-derived HBM bytes per launch of the dominant kernel, if profiled."""
+    """Counter-derived HBM bytes per launch of the 65536-point kernel, if profiled."""
     path = os.path.join(ROOT, "profiles", f"traffic_n{n}_b{batch}.json")
     if not os.path.exists(path):
         return None
@@ -87,256 +363,66 @@ def load_traffic(n: int, batch: int):
         return None
 
 
-def graph_time(torch, launch, reps):
-    """Mean ms per launch(i, stream) over reps, captured once into a hipGraph
-    and replayed between synchronizations."""
-    g = torch.cuda.CUDAGraph()
-    cs = torch.cuda.Stream()
-    cs.wait_stream(torch.cuda.current_stream())
-    with torch.cuda.graph(g, stream=cs):
-        for i in range(reps):
-            launch(i, cs.cuda_stream)
-    g.replay()
-    torch.cuda.synchronize()
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    a.record(torch.cuda.current_stream())
-    g.replay()
-    b.record(torch.cuda.current_stream())
-    torch.cuda.synchronize()
-    return a.elapsed_time(b) / reps
-
-
-def bench_split(args):
-    """Split precision, N = 2^16, FP32 complex in/out, N(0,1)*10^U(-3,3) per
-    signal; reports the time ratio to the plain fp16 exec of the same shape."""
-    import torch
-    import tcfft_amd
-    n, batch = 1 << 16, args.batch
-    rot = 4  # 4 x (128 MiB in + 128 MiB out) > MALL
-    rng = np.random.default_rng(0)
-    scale = 10.0 ** rng.uniform(-3, 3, (batch, 1))
-    x = ((rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n))) * scale).astype(np.complex64)
-    host = torch.from_numpy(x.view(np.float32))
-    ins = [host.cuda() for _ in range(rot)]
-    outs = [torch.empty_like(ins[0]) for _ in range(rot)]
-    plan = tcfft_amd.Plan1D(n, batch)
-    if args.split_chunk is not None:
-        plan.set_option(tcfft_amd.OPT_SPLIT_CHUNK, args.split_chunk)
-    plan.exec_split(ins[0].data_ptr(), outs[0].data_ptr())
-    torch.cuda.synchronize()
-    k = 8
-    got = outs[0][:k].cpu().numpy().view(np.complex64).astype(np.complex128)
-    ref = np.fft.fft(x[:k].astype(np.complex128), axis=-1)
-    errs = [float(np.linalg.norm(got[i] - ref[i]) / np.linalg.norm(ref[i])) for i in range(k)]
-    for i in range(args.warmup):
-        plan.exec_split(ins[i % rot].data_ptr(), outs[i % rot].data_ptr())
-    ms = graph_time(torch, lambda i, st: plan.exec_split(ins[i % rot].data_ptr(), outs[i % rot].data_ptr(),
-                                                         tcfft_amd.FORWARD, st), args.steps)
-    # plain fp16 exec of the same shape, same run
-    pbufs = [torch.empty((batch, n, 2), dtype=torch.float16, device="cuda") for _ in range(10)]
-    for b in pbufs:
-        b.uniform_(-1, 1)
-    ms_plain = graph_time(torch, lambda i, st: plan.exec(pbufs[i % 10].data_ptr(), tcfft_amd.FORWARD, st), args.steps)
-    flops = 5.0 * n * math.log2(n) * batch
-    out = {"metric": "split-precision FFT GFLOP/s (5N·log2N) at N=2^16; rel-L2 vs FP64 per signal",
-           "value": round(flops / (ms * 1e-3) / 1e9, 1), "unit": "GFLOP/s", "n_gpus": 1, "steps": args.steps,
-           "warmup": args.warmup, "ms_per_step": round(ms, 5), "higher_is_better": True, "scaling": "weak",
-           "vs_baseline": None, "dtype": "f16 hi+lo MFMA, FP32 accumulate, FP32 I/O",
-           "data": "synthetic N(0,1)*10^U(-3,3) per signal, complex64",
-           "config": {"workload": f"1D C2C split-precision FFT N={n} batch={batch}, FP32 in/out, natural order",
-                      "n": n, "batch": batch, "split_chunk": args.split_chunk},
-           "rel_l2_max_per_signal": max(errs), "ms_plain_fp16": round(ms_plain, 5),
-           "time_ratio_vs_plain": round(ms / ms_plain, 3),
-           "alg_bytes": 16 * n * batch, "hbm_frac_alg": round(16 * n * batch / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
-    plan.destroy()
-    print(json.dumps(out), flush=True)
-
-
-def bench_uneven(args):
-    """4096 fp16 signals, N = 2^U{8..16} (seeded), drop-in order, one
-    persistent autotuned launch vs one launch per size class (regular plans on
-    a size-sorted copy of the same signals)."""
-    import torch
-    import tcfft_amd
-    rng = np.random.default_rng(0)
-    n_sig = 4096
-    sizes = (2 ** rng.integers(8, 17, n_sig)).astype(np.int32)
-    total = int(sizes.sum())
-    flat = rng.uniform(-1, 1, (total, 2)).astype(np.float16)  # contents are synthetic per signal
-    mp = tcfft_amd.MultiPlan(sizes)
-    rot = 3  # 3 x 238 MB > MALL
-    host = torch.from_numpy(flat)
-    bufs = [host.cuda() for _ in range(rot)]
-    g, o, tune_ms = mp.autotune(bufs[0].data_ptr())
-    for b in bufs:
-        b.copy_(host)
-    ms_multi = graph_time(torch, lambda i, st: mp.exec(bufs[i % rot].data_ptr(), tcfft_amd.FORWARD, st), args.steps)
-    # per-size launches on a size-sorted layout of the same data volume
-    classes = sorted(set(int(n) for n in sizes))
-    plans, offs, o2 = [], [], 0
-    for n in classes:
-        cnt = int((sizes == n).sum())
-        plans.append(tcfft_amd.Plan1D(n, cnt))
-        offs.append(o2)
-        o2 += n * cnt
-
-    def per_size(i, st):
-        base = bufs[i % rot].data_ptr()
-        for p, off in zip(plans, offs):
-            p.exec(base + 4 * off, tcfft_amd.FORWARD, st)
-
-    ms_sizes = graph_time(torch, per_size, args.steps)
-    # accuracy of the persistent launch on a sample of signals (fresh data)
-    sample = [int(i) for i in rng.choice(n_sig, 16, replace=False)]
-    check = torch.from_numpy(flat).cuda()
-    xs = {}
-    for i in sample:
-        n, o3 = int(sizes[i]), int(mp.offsets[i])
-        rad, _ = tcfft_amd.query_1d(n, 1)
-        x = flat[o3:o3 + n].astype(np.float64)
-        xs[i] = x[:, 0] + 1j * x[:, 1]
-        xrev = flat[o3:o3 + n][tcfft_amd.gen_rev(n, rad)]
-        check[o3:o3 + n] = torch.from_numpy(np.ascontiguousarray(xrev)).cuda()
-    mp.exec(check.data_ptr())
-    torch.cuda.synchronize()
-    errs = []
-    for i in sample:
-        n, o3 = int(sizes[i]), int(mp.offsets[i])
-        y = check[o3:o3 + n].float().cpu().numpy().astype(np.float64)
-        yc = y[:, 0] + 1j * y[:, 1]
-        ref = np.fft.fft(xs[i])
-        errs.append(float(np.linalg.norm(yc - ref) / np.linalg.norm(ref)))
-    flops = float(sum(5.0 * n * math.log2(n) for n in sizes))
-    out = {"metric": "uneven-batch FFT GFLOP/s (sum 5N·log2N), one persistent launch",
-           "value": round(flops / (ms_multi * 1e-3) / 1e9, 1), "unit": "GFLOP/s", "n_gpus": 1,
-           "steps": args.steps, "warmup": 0, "ms_per_step": round(ms_multi, 5), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f16 (FP32 MFMA accumulate)",
-           "data": "synthetic uniform[-1,1) fp16, 4096 signals, N=2^U{8..16} seed 0",
-           "config": {"workload": "uneven batch, one persistent hipGraph-captured launch", "signals": n_sig,
-                      "elements": total, "tasks": mp.n_tasks, "autotuned_grid": g, "autotuned_order": o},
-           "ms_per_size_launches": round(ms_sizes, 5), "speedup_vs_per_size": round(ms_sizes / ms_multi, 3),
-           "rel_l2_max_sample": max(errs),
-           "hbm_frac_alg": round(8.0 * total / (ms_multi * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
-    for p in plans:
-        p.destroy()
-    mp.destroy()
-    print(json.dumps(out), flush=True)
-
-
 def main():
     args = parse()
-    if args.workload == "uneven":
-        if args.gpus != 1:
-            sys.exit(2)
-        bench_uneven(args)
-        return
-    if args.workload == "split":
-        if args.gpus != 1:
-            sys.exit(2)
-        bench_split(args)
-        return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batch axis shards with "
-              f"no exchange, see DESIGN.md. Run with --gpus 1.", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): every workload here shards on "
+              f"the batch axis with no exchange, and this build runs on one GPU. Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     import tcfft_amd
 
+    torch.cuda.set_device(0)
+    if args.workload != "plain":
+        leg = {"split": lambda: bench_split(torch, tcfft_amd, args.steps, args.split_chunk),
+               "uneven": lambda: bench_uneven(torch, tcfft_amd, args.steps),
+               "conv": lambda: bench_conv(torch, max(3, args.steps // 20))}[args.workload]()
+        print(json.dumps({"workload": args.workload, **leg}), flush=True)
+        return
+
     n, batch = args.n, args.batch
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
     plan = tcfft_amd.Plan1D(n, batch)
-    stream = torch.cuda.current_stream()
-    sptr = stream.cuda_stream
-
-    # synthetic fp16 inputs, digit-reversed by the plan's radices (gen_rev)
     rng = np.random.default_rng(0)
-    x = rng.uniform(-1, 1, (batch, n, 2)).astype(np.float16)
-    rev = tcfft_amd.gen_rev(n, plan.radices)
-    x_rev = np.ascontiguousarray(x[:, rev, :])
+    x = fp16_uniform(rng, (batch, n))
+    x_rev = np.ascontiguousarray(x[:, tcfft_amd.gen_rev(n, plan.radices)])
     host = torch.from_numpy(x_rev)
-    bufs = [torch.empty((batch, n, 2), dtype=torch.float16, device=dev) for _ in range(ROTATE)]
-    for b in bufs:
-        b.copy_(host)
+    bufs = rotation(torch, (batch, n, 2), torch.float16, host)
     torch.cuda.synchronize()
-
     if args.calib:
         for _ in range(5):
             bufs[1].copy_(bufs[0])
         torch.cuda.synchronize()
 
     # accuracy of one exec vs FP64 (numpy pocketfft) on a few signals
-    check = bufs[0]
-    plan.exec(check.data_ptr(), tcfft_amd.FORWARD, sptr)
+    plan.exec(bufs[0].data_ptr())
     torch.cuda.synchronize()
     k = min(batch, 8)
-    got = check[:k].float().cpu().numpy().astype(np.float64)
-    gotc = got[..., 0] + 1j * got[..., 1]
-    xin = x[:k].astype(np.float64)
-    ref = np.fft.fft(xin[..., 0] + 1j * xin[..., 1], axis=-1)
-    rel_l2 = float(np.linalg.norm(gotc - ref) / np.linalg.norm(ref))
-    check.copy_(host)
+    rel_l2 = rel(to_c(bufs[0][:k]), np.fft.fft(x[:k, :, 0].astype(np.float64) + 1j * x[:k, :, 1], axis=-1))
+    bufs[0].copy_(host)
 
-    def timed(launch, reps, graph):
-        """Mean ms per launch() over reps, bracketed by synchronize; with
-        graph=True the reps launches are captured once into a hipGraph and
-        replayed (no host launch overhead inside the timed region)."""
-        if graph:
-            g = torch.cuda.CUDAGraph()
-            cs = torch.cuda.Stream()
-            cs.wait_stream(torch.cuda.current_stream())
-            with torch.cuda.graph(g, stream=cs):
-                for i in range(reps):
-                    launch(i, cs.cuda_stream)
-            g.replay()  # untimed replay (replay() runs on the current stream)
-            torch.cuda.synchronize()
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record(torch.cuda.current_stream())
-            g.replay()
-            b.record(torch.cuda.current_stream())
-            torch.cuda.synchronize()
-            return a.elapsed_time(b) / reps
-        a = torch.cuda.Event(enable_timing=True)
-        b = torch.cuda.Event(enable_timing=True)
-        torch.cuda.synchronize()
-        a.record(stream)
-        for i in range(reps):
-            launch(i, sptr)
-        b.record(stream)
-        torch.cuda.synchronize()
-        return a.elapsed_time(b) / reps
-
-    def exec_step(i, st):
-        plan.exec(bufs[i % ROTATE].data_ptr(), tcfft_amd.FORWARD, st)
-
-    # warmup (eager), then the timed K steps: eager and graph-replayed
+    nb = len(bufs)
     for i in range(args.warmup):
-        exec_step(i, sptr)
+        plan.exec(bufs[i % nb].data_ptr())
     torch.cuda.synchronize()
     t_wall0 = time.perf_counter()
-    ms_eager = timed(exec_step, args.steps, graph=False)
-    ms = timed(exec_step, args.steps, graph=True)
+    ms_eager = eager_time(torch, lambda i: plan.exec(bufs[i % nb].data_ptr()), args.steps)
+    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % nb].data_ptr(), tcfft_amd.FORWARD, st), args.steps)
     t_wall = time.perf_counter() - t_wall0
 
     # per-pass timing (each pass kernel alone, graph-replayed on its stream)
-    npass = plan.num_passes
     per_pass = []
-    for p in range(npass):
-        def pass_step(i, st, p=p):
-            plan.exec_passes(bufs[i % ROTATE].data_ptr(), p, 1, tcfft_amd.FORWARD, st)
-        per_pass.append(timed(pass_step, max(20, args.steps), graph=True))
+    for p in range(plan.num_passes):
+        per_pass.append(graph_time(torch, lambda i, st, p=p: plan.exec_passes(bufs[i % nb].data_ptr(), p, 1,
+                                                                              tcfft_amd.FORWARD, st),
+                                   max(20, args.steps)))
     dom = int(np.argmax(per_pass))
     alg_bytes = 8.0 * n * batch
     achieved = alg_bytes / (per_pass[dom] * 1e-3) / 1e9
-    exec_gbs = alg_bytes / (ms * 1e-3) / 1e9
-
     flops = 5.0 * n * math.log2(n) * batch
-    gflops = flops / (ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
-        "value": round(gflops, 1),
+        "value": round(flops / (ms * 1e-3) / 1e9, 1),
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -348,21 +434,35 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f16 (FP32 MFMA accumulate)",
-        "data": "synthetic uniform[-1,1) fp16, resident in HBM, rotated over %d buffers" % ROTATE,
+        "data": f"synthetic uniform[-1,1) fp16, resident in HBM, rotated over {nb} buffers",
         "config": {"workload": f"1D C2C FFT N={n} batch={batch}, fp16 interleaved, in place, "
                                f"digit-reversed input (tcfftExec)", "n": n, "batch": batch,
-                   "passes": npass, "radices": plan.radices},
+                   "passes": plan.num_passes, "radices": plan.radices},
         "rel_l2_vs_fp64": rel_l2,
-        "hbm_frac_whole_exec": round(exec_gbs / PEAK_HBM_GBS, 4),
+        "hbm_frac_whole_exec": round(alg_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
         "pass_ms": [round(v, 5) for v in per_pass],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(n, batch),
-                     "kernel": f"pass {dom}"},
+                     "kernel": f"pass {dom} of {plan.num_passes}"},
         "wall_s_timed": round(t_wall, 3),
     }
+    plan.destroy()
+    del bufs
+    if not args.no_configs and n == N and batch == BATCH:
+        reps = max(20, args.steps // 4)
+        crng = np.random.default_rng(1)
+        cfg = {}
+        cfg["c1_n4096_b1024"] = bench_plan1d(torch, tcfft_amd, 4096, 1024, reps, crng)
+        cfg["c2a_n2^20_b16"] = bench_plan1d(torch, tcfft_amd, 1 << 20, 16, reps, crng)
+        cfg["c2b_2d_1024x1024_b8"] = bench_plan2d(torch, tcfft_amd, 1024, 1024, 8, reps, crng)
+        cfg["c3_split_n2^16_b256"] = bench_split(torch, tcfft_amd, reps, ms_plain=ms)
+        cfg["c4_uneven_4096_signals"] = bench_uneven(torch, tcfft_amd, reps)
+        cfg["c5_fftconv_64x256x8192"] = bench_conv(torch, 5)
+        cfg["stream_copy_tbs"] = stream_copy(torch)
+        out["configs"] = cfg
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
-    plan.destroy()
+        out["cpu_fft_f32"] = cpu_fft_f32(n, batch)
     print(json.dumps(out), flush=True)
 
 
