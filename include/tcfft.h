@@ -89,6 +89,35 @@ int tcfft_exec_natural(tcfft_plan plan, const void *in, void *out, int direction
  * plans only (else TCFFT_INVALID_SIZE). */
 int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
 
+/* Natural-order input, spectrum output in the plan's digit-reversed order
+ * (position p holds X[rev[p]], rev = tcfft_gen_rev of the plan's radices):
+ * the order tcfft_exec consumes, so a transform, a pointwise product in that
+ * order and an inverse tcfft_exec need no permutation.  1D plans with
+ * n = 4096, 8192 or 16384; in place or out of place.  No reference
+ * counterpart (the reference takes digit-reversed input only); it replaces
+ * the caller's host-side gen_rev staging (tcfft_doit_half.cpp:40-46) for
+ * spectral callers. */
+int tcfft_exec_dif(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
+
+/* Fused FFT convolution along the transform axis, one HBM round trip:
+ *   y[b][c] = F^H (H[c] * F x[b][c]) / n      (conj_filter = 0)
+ *   y[b][c] = F^H (conj(H[c]) * F x[b][c]) / n (conj_filter = 1, the data
+ *                                                gradient of the former)
+ * x, y: [batch/channels][channels][n] fp16 complex, natural order (x == y
+ * allowed); h_rev: [channels][n] fp16 complex spectrum in the digit-reversed
+ * order above; spec_out (nullable): receives F x, digit-reversed.  The plan
+ * is a 1D plan of n = 4096, 8192 or 16384 whose batch counts all signals.
+ * No reference counterpart (an FFT-conv layer built on the transform). */
+int tcfft_conv_exec(tcfft_plan plan, int channels, const void *x, const void *h_rev, void *y, int conj_filter,
+                    void *spec_out, void *stream);
+
+/* Filter gradient of tcfft_conv_exec: gh_rev[c][p] = (1/n) sum_b
+ * conj(X[b][c][p]) * G[b][c][p] from digit-reversed fp16 spectra of the
+ * input and of the output gradient; gh_rev is FP32 complex [channels][n],
+ * digit-reversed.  No reference counterpart. */
+int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, const void *g_spec, void *gh_rev,
+                           void *stream);
+
 /* Plan options. TCFFT_OPT_SPLIT_CHUNK: signals per pass pair of
  * tcfft_exec_split (0 = whole batch; default 32), which bounds the
  * intermediate's footprint so the second pass reads it from cache. */

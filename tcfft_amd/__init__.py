@@ -40,6 +40,7 @@ EXPORTS = (
     "tcfft_exec_split", "tcfft_plan_set_option",
     "tcfft_multi_create", "tcfft_multi_destroy", "tcfft_multi_info", "tcfft_multi_exec",
     "tcfft_multi_set_schedule", "tcfft_multi_autotune",
+    "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad",
 )
 OPT_SPLIT_CHUNK = 1
 
@@ -83,6 +84,9 @@ def lib():
         L.tcfft_exec_passes.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         L.tcfft_exec_split.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_plan_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.tcfft_exec_dif.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+        L.tcfft_conv_exec.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
+        L.tcfft_conv_filter_grad.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
         L.tcfft_multi_create.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, _i32p]
         L.tcfft_multi_destroy.argtypes = [_vp]
         L.tcfft_multi_info.argtypes = [_vp, _i32p, ctypes.POINTER(ctypes.c_longlong)]
@@ -168,6 +172,24 @@ class _Plan:
     def exec_split(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
         """FP32 complex in/out (natural order, in != out), FP16 hi/lo MFMA split precision."""
         _check(lib().tcfft_exec_split(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)), "tcfft_exec_split")
+
+    def exec_dif(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
+        """Natural-order input -> spectrum in the digit-reversed order of
+        self.radices (the order exec consumes); n = 4096, 8192 or 16384."""
+        _check(lib().tcfft_exec_dif(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)), "tcfft_exec_dif")
+
+    def conv_exec(self, channels: int, x_ptr: int, h_rev_ptr: int, y_ptr: int, conj_filter: bool = False,
+                  spec_ptr: int = 0, stream: int = 0):
+        """y = F^H (H * F x) / n per [.., channels, n] signal, H given as a
+        digit-reversed fp16 spectrum per channel; spec_ptr (optional)
+        receives F x, digit-reversed."""
+        _check(lib().tcfft_conv_exec(self._h, channels, _vp(x_ptr), _vp(h_rev_ptr), _vp(y_ptr), int(conj_filter),
+                                     _vp(spec_ptr) if spec_ptr else None, _vp(stream)), "tcfft_conv_exec")
+
+    def conv_filter_grad(self, channels: int, x_spec_ptr: int, g_spec_ptr: int, gh_rev_ptr: int, stream: int = 0):
+        """gh_rev[c] = (1/n) sum_b conj(X[b, c]) * G[b, c] (FP32, digit-reversed)."""
+        _check(lib().tcfft_conv_filter_grad(self._h, channels, _vp(x_spec_ptr), _vp(g_spec_ptr), _vp(gh_rev_ptr),
+                                            _vp(stream)), "tcfft_conv_filter_grad")
 
     def set_option(self, option: int, value: int):
         _check(lib().tcfft_plan_set_option(self._h, option, value), "tcfft_plan_set_option")

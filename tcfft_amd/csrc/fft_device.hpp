@@ -226,6 +226,9 @@ template <int T>
 __device__ __forceinline__ void small_dft(cf* v, int sign);
 
 template <>
+__device__ __forceinline__ void small_dft<1>(cf*, int) {}
+
+template <>
 __device__ __forceinline__ void small_dft<2>(cf* v, int) {
     cf a = v[0], b = v[1];
     v[0] = cadd(a, b);

@@ -16,121 +16,12 @@
 //                   per wave (B-form MFMA; T = 16 uses the permlane transpose).
 //   k_cols_small<R> strided pass of radix 2, 4 or 8 in FP32 VALU, four
 //                   adjacent columns per lane.
-#include "fft_device.hpp"
+#include "fft_blocks.hpp"
 #include "fft_kernels.hpp"
 
 #include <mutex>
 
 namespace tcfft {
-
-// Epilogue twiddle for the next pass: element at position P (within the
-// transform axis, P = global_pos >> shift) gets W_{s*R}^{c*rev(j)} with
-// c = P mod s, j = (P / s) mod R and rev the two-digit DIT reversal of the
-// next pass (radix ra first, then R/ra).  W_M^m = hi[m >> lo_bits]*lo[m & lo_mask].
-__device__ __forceinline__ cf epi_twiddle(const EpiParams& e, u32 gpos, float sgn) {
-    const u32 P = gpos >> e.shift;
-    const u32 c = P & ((1u << e.s_log2) - 1u);
-    const u32 j = (P >> e.s_log2) & ((1u << e.r_log2) - 1u);
-    const u32 j0 = j & ((1u << e.ra_log2) - 1u);
-    const u32 j1 = j >> e.ra_log2;
-    const u32 rj = j1 | (j0 << (e.r_log2 - e.ra_log2));
-    const u32 m = (c * rj) & e.m_mask;
-    const cf a = twiddle(e.hi, m >> e.lo_bits, sgn);
-    const cf b = twiddle(e.lo, m & ((1u << e.lo_bits) - 1u), sgn);
-    return cmul(a, b);
-}
-
-// Strided passes see their output at (block blk, row, column col) of a
-// buffer viewed as [blk][R][s]; along the axis that is P = blk*R*sa + row*sa
-// + xc with sa = s >> shift and xc = col >> shift, so the next pass's twiddle
-// is W_M^{(row*sa + xc)*rev(blk mod R')}: a per-column factor times a
-// per-row factor, both powers of one root.  epi_rev gives rev(blk mod R'),
-// epi_w gives W_M^m from the two-level table.
-__device__ __forceinline__ u32 epi_rev(const EpiParams& e, u32 blk) {
-    const u32 j = blk & ((1u << e.r_log2) - 1u);
-    const u32 j0 = j & ((1u << e.ra_log2) - 1u);
-    return (j >> e.ra_log2) | (j0 << (e.r_log2 - e.ra_log2));
-}
-
-__device__ __forceinline__ cf epi_w(const EpiParams& e, u32 m, float sgn) {
-    m &= e.m_mask;
-    return cmul(twiddle(e.hi, m >> e.lo_bits, sgn), twiddle(e.lo, m & ((1u << e.lo_bits) - 1u), sgn));
-}
-
-// ----------------------------------------------------------------------------
-// In-wave 4096-point DIT over the digits d0, d1, d2 of a 4096-element chunk.
-// In : d[t][j] = tile t (d2), lane (c16 = d1, g), word j (d0 = 4g + j).
-// Out: d[t][i] = tile t (k0), lane%16 = k1, (g, i) = k2, after post(t, i, v)
-//      is applied to every FP32 result of the last stage.
-// Stage-1 twiddle W_256^{k0*d1} is a per-lane constant; the stage-2 twiddle
-// W_4096^{(k0+16k1)*d2} is a per-lane recurrence over the tiles.  The
-// last stage uses the operand make_op2() (B-form), which may carry a row
-// scale; it is built after stage 1 so that it holds no registers before.
-// With REC0 the stage-2 inputs of column c = k0 + 16*k1 are also multiplied
-// by rec0[i] (k0 = 4g + i, k1 = lane%16): a per-column factor of the caller
-// merged into the stage-1 recurrence (rec = rec0 * W_4096^{c*d2}).
-// The per-lane twiddles come from the caller (Dit4096Tw), loaded before the
-// data: global loads complete in issue order, so a table load issued after
-// the data loads would hold stage 0 until the whole chunk has arrived.
-struct Dit4096Tw {
-    float2 tw1[4], step[4];  // raw table entries: W_256^{k0*d1}, W_4096^{k0+16*k1}
-};
-
-__device__ __forceinline__ void dit4096_tw_load(Dit4096Tw& tw, int lane, const float2* __restrict__ w4096) {
-    const int c16 = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        tw.tw1[i] = w4096[((c16 * (4 * g + i)) & 255) * 16];
-        tw.step[i] = w4096[16 * c16 + 4 * g + i];
-    }
-}
-
-template <bool REC0, class MakeOp2, class Post>
-__device__ __forceinline__ void dit4096(u32 (&d)[16][4], int sign, const Dit4096Tw& tw, const Dft16Op& op,
-                                        MakeOp2 make_op2, const cf (&rec0)[4], Post post) {
-    const float sg = sign < 0 ? 1.f : -1.f;
-    cf tw1[4], step[4], rec[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        tw1[i] = cf{tw.tw1[i].x, sg * tw.tw1[i].y};
-        step[i] = cf{tw.step[i].x, sg * tw.step[i].y};
-        rec[i] = REC0 ? rec0[i] : step[i];
-    }
-    // stage 0: DFT over d0 -> lane%16 = k0, (g,i) = d1
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
-        dft16_a(d[t], op, re, im);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
-    }
-    // stage 1: DFT over d1 -> lane%16 = k1, (g,i) = k0, tile = d2; twiddle W_4096^{c*d2}
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
-        dft16_a(d[t], op, re, im);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            cf v = {re[i], im[i]};
-            if (REC0 || t) {
-                v = cmul(v, rec[i]);
-                if (t < 15) rec[i] = cmul(rec[i], step[i]);
-            }
-            d[t][i] = pack(v);
-        }
-    }
-    u32 e[16][4];
-    transpose_gi_tile(d, e);  // (g,i) = d2, tile = k0
-    const Dft16Op op2 = make_op2();
-    // stage 2 (B-form): columns k1, points d2 -> (g,i) = k2
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
-        dft16_b(e[t], op2, re, im);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) d[t][i] = pack(post(t, i, cf{re[i], im[i]}));
-    }
-}
 
 // ----------------------------------------------------------------------------
 // k_rows<T>: lines of L = 256*T contiguous elements, digit-reversed for the
@@ -442,183 +333,6 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) store4(buf + base + (size_t)j * s, w[j]);
-}
-
-// ----------------------------------------------------------------------------
-// k_big<W>: one signal of N = 4096*W (W = 2, 4, 8, 16) per workgroup of W
-// waves, in a single HBM round trip.  Radix list [16, 16, 16, W]; position
-// p = d0 + 16*d1 + 256*d2 + 4096*w, wave w loads chunk w.
-//  phase 1: each wave does the 4096-point DIT of its chunk in registers and
-//           multiplies by the final-stage twiddle W_N^{w*c}, c = k0+16k1+256k2:
-//           W_N^{256*w*k2} is folded into the stage-2 operand (row scale),
-//           W_N^{w*(k0+16k1)} is a per-lane recurrence over the tiles.
-//  phase 2: LDS exchange (layout [w][sigma(c)], b128 both ways) and the
-//           radix-W stage over w (MFMA for W = 16, VALU for W = 2, 4, 8);
-//           output position c + 4096*k3.  Two rounds of 2048 columns for
-//           W >= 8 keep the exchange at W*8 KiB of LDS.
-// sigma(c) = c ^ ((c >> 4) & 15) << 2 is a bijection that keeps 16-lane
-// groups conflict-free both for writers (lane%16 = c bits 4-7) and for
-// readers (lane%16 = c bits 2-5); readers own 4 adjacent columns per lane so
-// the final global stores are 256-byte runs per 16 lanes.
-__device__ __forceinline__ u32 lds_sigma(u32 c) { return c ^ (((c >> 4) & 15u) << 2); }
-
-template <int W, int ROUNDS_ = (W >= 8 ? 2 : 1)>
-struct BigCfg {
-    static constexpr int ROUNDS = ROUNDS_;
-    static constexpr int COLS = 4096 / ROUNDS;  // columns exchanged per round
-    static constexpr int LDS_WORDS = W * COLS;
-    static constexpr u32 N = 4096u * W;
-};
-
-// One signal of N = 4096*W at sbase, processed by the W waves of a group
-// (this wave is number w) with lds = the group's W*COLS words of LDS.
-// Every barrier is a workgroup barrier: all groups of a workgroup run the
-// same sequence.  sig is the signal's index (epilogue positions only).
-template <int W, int ROUNDS, bool EPI, int SIGN>
-__device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* lds, u32 sig,
-                                         const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
-                                         const float2* __restrict__ wn_lo, u32 wn_lo_bits, const EpiParams& epi,
-                                         bool active = true) {
-    static_assert(!EPI || W == 16, "the epilogue twiddle is folded for W = 16 only");
-    constexpr int sign = SIGN;
-    using C = BigCfg<W, ROUNDS>;
-    const int lane = lane_id();
-    const int c16 = lane & 15, g = lane >> 4;
-    const float sg = sign < 0 ? 1.f : -1.f;
-
-    // Phase-1 twiddle W_N^{w*c} = W_Q^{c*e} with root W_Q and exponent e:
-    //   no epilogue: Q = N, e = w;
-    //   epilogue (W = 16, next pass of radix R = 2^r at step N over chunks
-    //   j = sig mod R): the next pass's twiddle W_{NR}^{(c + 4096k)*rev(j)}
-    //   splits into W_{NR}^{c*rev(j)}, merged here as Q = N*R,
-    //   e = w*R + rev(j), and W_{NR}^{4096*k*rev(j)}, a row scale of the
-    //   final-stage operand (output frequency k).
-    u32 e_w = (u32)w, rj = 0u;
-    if constexpr (EPI) {
-        const u32 j = sig & ((1u << epi.r_log2) - 1u);
-        const u32 j0 = j & ((1u << epi.ra_log2) - 1u);
-        rj = (j >> epi.ra_log2) | (j0 << (epi.r_log2 - epi.ra_log2));
-        e_w = ((u32)w << epi.r_log2) + rj;
-    }
-    // W_Q^m = hi[m >> lo_bits] * lo[m & (2^lo_bits - 1)] (two-level table of Q)
-    const float2* q_hi = EPI ? epi.hi : wn_hi;
-    const float2* q_lo = EPI ? epi.lo : wn_lo;
-    const u32 q_bits = EPI ? epi.lo_bits : wn_lo_bits;
-    const u32 q_mask = EPI ? epi.m_mask : C::N - 1u;
-    struct RawQ {
-        float2 h, l;
-    };
-    auto wq_load = [&](u32 m) -> RawQ {
-        m &= q_mask;
-        return RawQ{q_hi[m >> q_bits], q_lo[m & ((1u << q_bits) - 1u)]};
-    };
-    auto wq_val = [&](const RawQ& r) -> cf { return cmul(cf{r.h.x, sg * r.h.y}, cf{r.l.x, sg * r.l.y}); };
-
-    // Table loads first, then the data: global loads complete in issue
-    // order, so table values loaded after the data would hold the first
-    // stage until the whole signal has arrived.
-    //   rec0: W_Q^{e*(k0 + 16*k1)} (k0 = 4g + i, k1 = lane%16), constant over
-    //         k2 for c = k0 + 16*k1 + 256*k2, merged into the stage-1 recurrence;
-    //   s2:   W_Q^{e*256*k2} as a row scale of the stage-2 operand (f = k2);
-    //   s3:   the final-stage row scale under the epilogue.
-    Dit4096Tw tw;
-    dit4096_tw_load(tw, lane, w4096);
-    RawQ r_rec0[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) r_rec0[i] = wq_load(e_w * (4u * g + i + 16u * c16));
-    const RawQ r_s2 = wq_load(256u * e_w * (u32)c16);
-    const RawQ r_s3 = wq_load(4096u * rj * (u32)c16);
-    __builtin_amdgcn_sched_barrier(0);
-    u32 d[16][4];
-    {
-        const u32* src = sbase + 4096 * w + 16 * c16 + 4 * g;
-#pragma unroll
-        for (int t = 0; t < 16; ++t) load4_nt(d[t], src + 256 * t);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-
-    const Dft16Op op = make_dft16(lane, sign);
-    cf rec0[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) rec0[i] = wq_val(r_rec0[i]);
-    const cf s2 = wq_val(r_s2);
-    auto make_op2 = [&] { return make_dft16_scaled(lane_id(), sign, s2); };
-    dit4096<true>(d, sign, tw, op, make_op2, rec0, [](int, int, cf v) -> cf { return v; });
-    // d: tile = k0, lane%16 = k1, (g,i) = k2.  Writer layout: word = c bits 0-1
-    u32 e[16][4];
-    transpose_gi_tile(d, e);  // (g,i) = k0, tile = k2: c = (4g+i) + 16*c16 + 256*t
-    // final-stage operand (W = 16): rows k scaled by W_Q^{4096*k*rev(j)} under
-    // the epilogue; built after the first exchange barrier (a fresh lane id)
-    // so that it does not hold registers during the in-wave DIT
-    Dft16Op op3 = op;
-
-#pragma unroll
-    for (int r = 0; r < C::ROUNDS; ++r) {
-        constexpr int TPR = 16 / C::ROUNDS;  // tiles per round
-        if (r) __syncthreads();              // previous round's reads are done
-#pragma unroll
-        for (int tt = 0; tt < TPR; ++tt) {
-            const u32 off = (u32)w * C::COLS + lds_sigma(4u * g + 16u * c16 + 256u * tt);
-            store4(lds + off, e[r * TPR + tt]);
-        }
-        __syncthreads();
-        if constexpr (EPI) {
-            if (r == 0) op3 = make_dft16_scaled(lane_id(), sign, wq_val(r_s3));
-        }
-        if constexpr (W == 16) {
-            // reader wave w': columns c = j + 4*c16 + 64h + 128w' (+2048r)
-            u32 v[2][4][4];  // [h][i = point (w = 4g+i)][j]
-#pragma unroll
-            for (int h = 0; h < 2; ++h)
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const u32 off = (u32)(4 * g + i) * C::COLS + lds_sigma(4u * c16 + 64u * h + 128u * w);
-                    load4(v[h][i], lds + off);
-                }
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                u32 o[4][4];  // [j][i' = k3 word]
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    u32 a[4] = {v[h][0][j], v[h][1][j], v[h][2][j], v[h][3][j]};
-                    f32x4 re, im;
-                    dft16_b(a, op3, re, im);
-#pragma unroll
-                    for (int i = 0; i < 4; ++i) o[j][i] = pack(cf{re[i], im[i]});
-                }
-                const u32 c0 = 2048u * r + 4u * c16 + 64u * h + 128u * w;
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    u32 q[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
-                    if (active) store4_nt(sbase + c0 + 4096u * (4u * g + i), q);
-                }
-            }
-        } else {
-            // reader wave w': columns c = j + 4*lane + 256*(q + QN*w') (+2048r)
-            constexpr int QN = C::COLS / (256 * W);
-#pragma unroll
-            for (int q = 0; q < QN; ++q) {
-                u32 v[W][4];
-#pragma unroll
-                for (int s = 0; s < W; ++s)
-                    load4(v[s], lds + (u32)s * C::COLS + lds_sigma(4u * lane + 256u * (q + QN * w)));
-                u32 o[W][4];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    cf x[W];
-#pragma unroll
-                    for (int s = 0; s < W; ++s) x[s] = unpack(v[s][j]);
-                    small_dft<W>(x, sign);
-#pragma unroll
-                    for (int s = 0; s < W; ++s) o[s][j] = pack(x[s]);
-                }
-                const u32 c0 = 2048u * r + 4u * lane + 256u * (q + QN * w);
-#pragma unroll
-                for (int s = 0; s < W; ++s)
-                    if (active) store4_nt(sbase + c0 + 4096u * s, o[s]);
-            }
-        }
-    }
 }
 
 template <int W, bool EPI, int SIGN>

@@ -47,6 +47,20 @@ struct MultiTables {  // two-level W_N tables for N = 2^13 .. 2^16
 hipError_t launch_multi(uint32_t* buf, const MultiTask* tasks, uint32_t n_tasks, const uint32_t* offsets,
                         uint32_t grid, int sign, const float2* w4096, const MultiTables& tb, hipStream_t st);
 
+// Natural-order transform of N = 4096*W (W = 1, 2, 4) per signal whose output
+// is the spectrum in the digit-reversed order the DIT kernels consume.
+hipError_t launch_dif(int W, const uint32_t* in, uint32_t* out, uint32_t n_signals, int sign, const float2* w4096,
+                      const float2* wn_hi, const float2* wn_lo, uint32_t lo_bits, hipStream_t st);
+// Fused FFT convolution y = F^H (H * F x) * scale (N = 4096*W, W = 1, 2, 4);
+// h_rev digit-reversed fp16 spectrum per channel, conj -> conj(H); spec
+// (nullable) receives F x, digit-reversed.
+hipError_t launch_conv(int W, bool conj, const uint32_t* x, const uint32_t* h_rev, uint32_t* y, uint32_t* spec,
+                       uint32_t n_signals, uint32_t channels, float scale, const float2* w4096, const float2* wn_hi,
+                       const float2* wn_lo, uint32_t lo_bits, hipStream_t st);
+// gh_rev[c][p] = scale * sum_b conj(X[b][c][p]) G[b][c][p] (FP32 out).
+hipError_t launch_conv_wgrad(const uint32_t* xs, const uint32_t* gs, float2* gh, uint32_t n, uint32_t channels,
+                             uint32_t batch, float scale, hipStream_t st);
+
 // Strided pass of radix R (2..256) at element step 2^s_log2 over `outer` blocks.
 hipError_t launch_cols(int R, uint32_t* buf, uint32_t s_log2, uint32_t outer, int sign, const float2* w4096,
                        const EpiParams* epi, hipStream_t st);

@@ -424,6 +424,87 @@ int tcfft_exec_split(tcfft_plan plan, const void* in, void* out, int direction, 
     return TCFFT_SUCCESS;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Parameters of the natural-order / convolution kernels for a 1D plan of
+// n = 4096*W (W = 1, 2, 4): W and the two-level W_n table (W = 1 needs only
+// its entry 0, which the 4096-point table provides).
+int dif_params(tcfft_plan_s* p, int& W, const float2*& hi, const float2*& lo, uint32_t& bits) {
+    const Layout& L = p->lay;
+    if (L.dims != 1 || (L.n != 4096 && L.n != 8192 && L.n != 16384)) return TCFFT_INVALID_SIZE;
+    W = L.n / 4096;
+    if (W == 1) {
+        hi = lo = p->w4096;
+        bits = 6;
+        return TCFFT_SUCCESS;
+    }
+    const Pass& ps = L.passes[0];
+    if (ps.kind != 2 || ps.radix != W) return TCFFT_INVALID_SIZE;
+    hi = ps.big_hi;
+    lo = ps.big_lo;
+    bits = ps.big_M_log2 / 2;
+    return TCFFT_SUCCESS;
+}
+
+bool aligned16(const void* ptr) { return ptr && !((uintptr_t)ptr & 15u); }
+
+}  // namespace
+
+extern "C" {
+
+int tcfft_exec_dif(tcfft_plan plan, const void* in, void* out, int direction, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!aligned16(in) || !aligned16(out)) return TCFFT_INVALID_POINTER;
+    if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
+    int W = 0;
+    const float2 *hi = nullptr, *lo = nullptr;
+    uint32_t bits = 0;
+    int rc = dif_params(plan, W, hi, lo, bits);
+    if (rc) return rc;
+    DeviceGuard guard(plan->device);
+    hipError_t e = tcfft::launch_dif(W, (const uint32_t*)in, (uint32_t*)out, (uint32_t)plan->lay.batch, direction,
+                                     plan->w4096, hi, lo, bits, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_conv_exec(tcfft_plan plan, int channels, const void* x, const void* h_rev, void* y, int conj_filter,
+                    void* spec_out, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!aligned16(x) || !aligned16(h_rev) || !aligned16(y)) return TCFFT_INVALID_POINTER;
+    if (spec_out && (!aligned16(spec_out) || spec_out == x || spec_out == y)) return TCFFT_INVALID_POINTER;
+    if (conj_filter != 0 && conj_filter != 1) return TCFFT_INVALID_VALUE;
+    int W = 0;
+    const float2 *hi = nullptr, *lo = nullptr;
+    uint32_t bits = 0;
+    int rc = dif_params(plan, W, hi, lo, bits);
+    if (rc) return rc;
+    if (channels < 1 || plan->lay.batch % channels) return TCFFT_INVALID_BATCH;
+    DeviceGuard guard(plan->device);
+    hipError_t e = tcfft::launch_conv(W, conj_filter == 1, (const uint32_t*)x, (const uint32_t*)h_rev, (uint32_t*)y,
+                                      (uint32_t*)spec_out, (uint32_t)plan->lay.batch, (uint32_t)channels,
+                                      1.0f / (float)plan->lay.n, plan->w4096, hi, lo, bits, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void* x_spec, const void* g_spec, void* gh_rev,
+                           void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!aligned16(x_spec) || !aligned16(g_spec) || !aligned16(gh_rev)) return TCFFT_INVALID_POINTER;
+    const Layout& L = plan->lay;
+    if (L.dims != 1) return TCFFT_INVALID_SIZE;
+    if (channels < 1 || L.batch % channels) return TCFFT_INVALID_BATCH;
+    DeviceGuard guard(plan->device);
+    hipError_t e = tcfft::launch_conv_wgrad((const uint32_t*)x_spec, (const uint32_t*)g_spec, (float2*)gh_rev,
+                                            (uint32_t)L.n, (uint32_t)channels, (uint32_t)(L.batch / channels),
+                                            1.0f / (float)L.n, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
     if (!valid(plan)) return TCFFT_INVALID_PLAN;
     switch (option) {
