@@ -15,13 +15,15 @@ library or a GPU is missing.
 
 FFTConv1d implements y = F^H (H * F x) / n with a learnable filter spectrum H
 broadcast over leading dimensions (circular convolution by the impulse
-response F^H H / n).
+response F^H H / n); for n = 4096, 8192 and 16384 forward and backward run
+the fused single-pass convolution kernel (tcfft_conv_exec), which keeps the
+spectrum in registers and in the drop-in's digit-reversed order.
 """
 from __future__ import annotations
 
 import torch
 
-from . import FORWARD, INVERSE, Plan1D
+from . import FORWARD, INVERSE, Plan1D, gen_rev
 
 _PLANS: dict = {}
 
@@ -99,11 +101,72 @@ def complex_mul(a: torch.Tensor, h: torch.Tensor) -> torch.Tensor:
     return torch.stack([ar * hr - ai * hi, ar * hi + ai * hr], dim=-1).to(a.dtype)
 
 
+FUSED_N = (4096, 8192, 16384)
+_REV: dict = {}
+
+
+def _rev(n: int, device: torch.device):
+    """(rev, inverse rev) index tensors of the digit-reversed spectrum order."""
+    key = (n, device.index)
+    r = _REV.get(key)
+    if r is None:
+        plan = _plan(n, 1, device)
+        rev = torch.as_tensor(gen_rev(n, plan.radices).astype("int64"), device=device)
+        r = (rev, torch.argsort(rev))
+        _REV[key] = r
+    return r
+
+
+class _FusedConv(torch.autograd.Function):
+    """y = F^H (H * F x) / n with the single-pass HIP kernel; backward:
+    gx = F^H (conj(H) * F gy) / n (same kernel, conjugated filter) and
+    gH = (1/n) sum_b conj(F x_b) * F gy_b from the spectra both passes store."""
+
+    @staticmethod
+    def forward(ctx, x, weight):
+        _check(x)
+        x = x.contiguous()
+        n, C = x.shape[-2], weight.shape[0]
+        signals = x.numel() // (2 * n)
+        rev, irev = _rev(n, x.device)
+        h_rev = weight.detach().index_select(1, rev).half().contiguous()
+        plan = _plan(n, signals, x.device)
+        stream = torch.cuda.current_stream(x.device).cuda_stream
+        y = torch.empty_like(x)
+        # the spectrum of x is kept only when the filter gradient will be needed
+        spec = torch.empty_like(x) if ctx.needs_input_grad[1] else None
+        plan.conv_exec(C, x.data_ptr(), h_rev.data_ptr(), y.data_ptr(), False,
+                       spec.data_ptr() if spec is not None else 0, stream)
+        ctx.save_for_backward(h_rev, spec if spec is not None else torch.empty(0, device=x.device))
+        ctx.shape = (n, C, signals, spec is not None)
+        return y
+
+    @staticmethod
+    def backward(ctx, gy):
+        h_rev, spec = ctx.saved_tensors
+        n, C, signals, want_h = ctx.shape
+        gy = gy.to(torch.float16).contiguous()
+        plan = _plan(n, signals, gy.device)
+        stream = torch.cuda.current_stream(gy.device).cuda_stream
+        gx = torch.empty_like(gy)
+        gspec = torch.empty_like(gy) if want_h else None
+        plan.conv_exec(C, gy.data_ptr(), h_rev.data_ptr(), gx.data_ptr(), True,
+                       gspec.data_ptr() if gspec is not None else 0, stream)
+        gw = None
+        if want_h:
+            gh_rev = torch.empty(C, n, 2, dtype=torch.float32, device=gy.device)
+            plan.conv_filter_grad(C, spec.data_ptr(), gspec.data_ptr(), gh_rev.data_ptr(), stream)
+            gw = gh_rev.index_select(1, _rev(n, gy.device)[1])
+        return gx, gw
+
+
 class FFTConv1d(torch.nn.Module):
     """y = F^H (H * F x) / n along the n axis of an fp16 [..., C, n, 2] input.
 
     H has shape [C, n, 2] (a complex filter spectrum per channel) and is kept
-    in fp32 as a parameter; it is cast per call.
+    in fp32 as a parameter.  For n = 4096, 8192, 16384 the layer runs the
+    fused single-pass convolution kernel (forward and both gradients);
+    other sizes compose the differentiable transforms with an fp32 product.
     """
 
     def __init__(self, channels: int, n: int, std: float = 0.02, device=None):
@@ -112,6 +175,8 @@ class FFTConv1d(torch.nn.Module):
         self.weight = torch.nn.Parameter(torch.randn(channels, n, 2, device=device) * std)
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
+        if self.n in FUSED_N:
+            return _FusedConv.apply(x, self.weight)
         X = fft(x)
         Y = complex_mul(X, self.weight)
         y = fft_adjoint(Y)

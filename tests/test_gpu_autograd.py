@@ -81,10 +81,12 @@ def reference_conv(x_c, h_c, n):
     return np.fft.ifft(np.fft.fft(x_c, axis=-1) * h_c, axis=-1)
 
 
-def test_fftconv_gradients_vs_fp64_finite_differences():
-    note("[gpu] fftconv finite differences")
+@pytest.mark.parametrize("n", [256, 4096])
+def test_fftconv_gradients_vs_fp64_finite_differences(n):
+    """n = 256 composes the transforms; n = 4096 runs the fused kernel."""
+    note(f"[gpu] fftconv finite differences n={n}")
     torch.manual_seed(0)
-    B, C, n = 2, 3, 256
+    B, C = 2, 3
     layer = torch_fft.FFTConv1d(C, n, std=0.5, device="cuda")
     x = fp16_pair((B, C, n), 4).cuda().requires_grad_(True)
     w = torch.randn(B, C, n, 2, device="cuda")
