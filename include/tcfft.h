@@ -119,7 +119,7 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
                            void *stream);
 
 /* Plan options. TCFFT_OPT_SPLIT_CHUNK: signals per pass pair of
- * tcfft_exec_split (0 = whole batch; default 32), which bounds the
+ * tcfft_exec_split (0 = whole batch; default 128), which bounds the
  * intermediate's footprint so the second pass reads it from cache. */
 enum tcfft_option { TCFFT_OPT_SPLIT_CHUNK = 1 };
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);

@@ -31,6 +31,7 @@ typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef u32 u32x4 __attribute__((ext_vector_type(4)));
+typedef u32 u32x2 __attribute__((ext_vector_type(2)));
 
 struct cf {  // FP32 complex value in registers
     float x, y;
@@ -194,6 +195,20 @@ __device__ __forceinline__ void transpose_gi_tile(u32 (&src)[16][4], u32 (&dst)[
         if (!(t & 8))
 #pragma unroll
             for (int i = 0; i < 4; ++i) swap_l5(dst[t][i], dst[t | 8][i]);
+}
+
+// The same transpose on FP32 data (register renames and lane swaps only).
+__device__ __forceinline__ void transpose_gi_tile(float (&src)[16][4], float (&dst)[16][4]) {
+    u32 a[16][4], b[16][4];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a[t][i] = __float_as_uint(src[t][i]);
+    transpose_gi_tile(a, b);
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) dst[t][i] = __uint_as_float(b[t][i]);
 }
 
 // Streaming (non-temporal) variants for data touched once per pass.

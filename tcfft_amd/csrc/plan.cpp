@@ -183,7 +183,7 @@ struct tcfft_plan_s {
     void* workspace = nullptr;
     size_t workspace_bytes = 0;
     size_t total = 0;  // elements in the buffer
-    int split_chunk = 32;  // signals per pass pair in tcfft_exec_split
+    int split_chunk = 128;  // signals per pass pair in tcfft_exec_split
 };
 
 namespace {

@@ -2,22 +2,26 @@
 //
 // Every radix-16 stage computes D = Fh*Xh + Fh*Xl + Fl*Xh with
 // v_mfma_f32_16x16x32_f16 and FP32 accumulation, where X = Xh + Xl is the
-// FP16 hi/lo split of the (scaled) FP32 data and F = Fh + Fl that of the
+// FP16 hi/lo split of the scaled FP32 data and F = Fh + Fl that of the
 // DFT-16 matrix.  The dropped Fl*Xl term and the two FP16 roundings leave
-// about 2^-22 relative error per stage.  Before each split the 16 points of
-// an MFMA column are scaled by a power of two so their largest component is
-// below 2^10 (no subnormal lo parts, no overflow); the scale is undone on the
-// FP32 result, exactly.  Twiddles come from FP64-generated tables.
+// about 2^-22 relative error per stage.  Each wave scales its data by one
+// power of two so that the largest component is below 2^13 (no overflow in
+// FP16; values far below the largest lose only bits that are negligible next
+// to it in the L2 norm); the bound carries to the second stage (a DFT-16
+// grows magnitudes at most 16-fold, so that stage uses scale/16) and the
+// inverse scale is applied to the FP32 results.  Twiddles come from
+// FP64-generated tables.
 //
-// N = 65536 = 256 x 256 as a natural-order four-step in two passes over
-// FP32 data, each pass a workgroup-per-(signal, 16 columns) kernel staging a
-// 256 x 16 tile in LDS:
-//   pass 1: x[c + 256 r] -> Y_c[k] = sum_r x W_256^{rk}, times W_N^{c k},
-//           stored transposed at out[k + 256 c]
-//   pass 2: out[k + 256 c] -> X[k + 256 k'] = sum_c Y'_c[k] W_256^{c k'},
-//           in place
-// Inside a pass the 256-point DFT is 16 x 16:
-//   X[k0 + 16 k1] = sum_r0 W_16^{r0 k1} W_256^{r0 k0} sum_r1 x[r0 + 16 r1] W_16^{r1 k0}.
+// N = 65536 = 256 x 256 as a four-step in two passes over FP32 data, both
+// the same kernel: each wave computes the 256-point DFTs down 16 adjacent
+// columns of a 256 x 256 matrix, entirely in registers (no LDS):
+//   pass 1: x[c + 256 r] -> Y_c[k] = W_N^{c k} sum_r x W_256^{r k},
+//           stored transposed as T[k + 256 c] (4 consecutive k per lane,
+//           128-byte runs per row);
+//   pass 2: T[k + 256 c] -> X[k + 256 m] = sum_c T W_256^{c m}.
+// Column DFT-256 over rows r = r0 + 16 r1, output k = k0 + 16 k1:
+//   stage 1 (tile r0): B-form over r1 -> rows k0; twiddle W_256^{r0 k0};
+//   transpose (g,i) <-> tile; stage 2 (tile k0): B-form over r0 -> rows k1.
 #include "fft_device.hpp"
 #include "fft_kernels.hpp"
 
@@ -50,22 +54,17 @@ __device__ __forceinline__ Dft16Split make_dft16_split(int lane, int sign) {
     return s;
 }
 
-// DFT-16 (B-form) of the lane's four points v, scaled per MFMA column.
-// Returns the FP32 results for frequencies 4g+i of column lane%16.
-__device__ __forceinline__ void dft16_split(const cf* v, const Dft16Split& F, cf* out) {
-    float m = 0.f;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) m = fmaxf(m, fmaxf(fabsf(v[j].x), fabsf(v[j].y)));
-    m = fmaxf(m, __shfl_xor(m, 16));
-    m = fmaxf(m, __shfl_xor(m, 32));
-    int e = 0;
-    (void)frexpf(m, &e);  // m < 2^e
-    const float s = m > 0.f ? ldexpf(1.f, 10 - e) : 1.f;
-    const float inv = m > 0.f ? ldexpf(1.f, e - 10) : 1.f;
+// B-form DFT-16 of the four (scaled) points (re[j], im[j]) * s of this lane
+// (column lane%16, rows 4g+j): FP32 results for rows 4g+i in (ore, oim).
+template <bool SCALE>
+__device__ __forceinline__ void dft16_split_b(const float* re, const float* im, float s, const Dft16Split& F,
+                                              f32x4& ore, f32x4& oim) {
     u32 hi[4], lo[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-        const float xr = v[j].x * s, xi = v[j].y * s;
+        // hi = fp16(x*s); lo = fp16(x*s - hi) (the difference is exact in
+        // FP32); without SCALE, s = 1
+        const float xr = SCALE ? re[j] * s : re[j], xi = SCALE ? im[j] * s : im[j];
         hi[j] = pack(xr, xi);
         const cf h = unpack(hi[j]);
         lo[j] = pack(xr - h.x, xi - h.y);
@@ -73,117 +72,174 @@ __device__ __forceinline__ void dft16_split(const cf* v, const Dft16Split& F, cf
     const f16x8 bh = as_operand(hi[0], hi[1], hi[2], hi[3]);
     const f16x8 bl = as_operand(lo[0], lo[1], lo[2], lo[3]);
     const f32x4 z = {0.f, 0.f, 0.f, 0.f};
-    f32x4 re = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.h.re, bh, z, 0, 0, 0);
-    re = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.h.re, bl, re, 0, 0, 0);
-    re = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.l.re, bh, re, 0, 0, 0);
-    f32x4 im = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.h.im, bh, z, 0, 0, 0);
-    im = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.h.im, bl, im, 0, 0, 0);
-    im = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.l.im, bh, im, 0, 0, 0);
-#pragma unroll
-    for (int i = 0; i < 4; ++i) out[i] = cf{re[i] * inv, im[i] * inv};
+    ore = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.h.re, bh, z, 0, 0, 0);
+    ore = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.h.re, bl, ore, 0, 0, 0);
+    ore = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.l.re, bh, ore, 0, 0, 0);
+    oim = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.h.im, bh, z, 0, 0, 0);
+    oim = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.h.im, bl, oim, 0, 0, 0);
+    oim = __builtin_amdgcn_mfma_f32_16x16x32_f16(F.l.im, bh, oim, 0, 0, 0);
 }
 
-constexpr int SPLIT_LDS_STRIDE = 17;  // float2 per LDS row (16 columns + 1 pad)
+// 2^(13 - e) for the wave-wide largest magnitude m = f * 2^e (f in [0.5, 1)),
+// and its inverse; 1 for m = 0.
+__device__ __forceinline__ void wave_scale(float m, float& s, float& inv) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d));
+    const int ex = (int)((__float_as_uint(m) >> 23) & 255u);  // m in [2^(ex-127), 2^(ex-126))
+    if (ex == 0) {
+        s = inv = 1.f;
+        return;
+    }
+    const int sb = min(max(266 - ex, 1), 254), ib = min(max(ex - 12, 1), 254);
+    s = __uint_as_float((u32)sb << 23);
+    inv = __uint_as_float((u32)ib << 23);
+}
 
-template <int MODE, int SIGN>
-__global__ __launch_bounds__(256) void k_split256(const float2* __restrict__ src, float2* __restrict__ dst,
-                                                  const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
-                                                  const float2* __restrict__ wn_lo) {
-    constexpr int sign = SIGN;
-    constexpr u32 N = 65536;
-    __shared__ float2 tile[256 * SPLIT_LDS_STRIDE];
-    const int tid = threadIdx.x;
-    const int lane = tid & 63, w = tid >> 6;
+template <int PASS, int SIGN>
+__global__ __launch_bounds__(256, 2) void k_split_col(const float2* __restrict__ src, float2* __restrict__ dst,
+                                                   const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
+                                                   const float2* __restrict__ wn_lo) {
+    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
+    const int lane = lane_id(), w = threadIdx.x >> 6;
     const int c16 = lane & 15, g = lane >> 4;
-    const float sg = sign < 0 ? 1.f : -1.f;
-    const u32 b = blockIdx.x >> 4, q = blockIdx.x & 15;
-    const size_t sbase = (size_t)b * N;
+    const u32 sig = blockIdx.x >> 2;
+    const u32 col = 64u * (blockIdx.x & 3u) + 16u * w + c16;
+    const size_t sbase = (size_t)sig * 65536u;
 
-    // load 256 rows x 16 columns (columns 16q..16q+15, row stride 256)
+    // the stage-1 twiddle step W_256^{4g+i} first (global loads complete in
+    // issue order, so a table value loaded after the data would wait for it)
+    float2 t_step[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t_step[i] = w4096[16 * (4 * g + i)];
+    __builtin_amdgcn_sched_barrier(0);
+    // rows r = r0 + 16 (4g + j): re/im of tile r0, word j
+    float dr[16][4], di[16][4];
     {
-        const float2* in = (MODE == 1 ? src : dst) + sbase + 16u * q;
+        const float2* p = src + sbase + col + 16384u * g;  // row 64g, column col
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int row = (tid >> 3) + 32 * j, cp = tid & 7;
-            const f32x4* p = reinterpret_cast<const f32x4*>(in + (size_t)row * 256 + 2 * cp);
-            const f32x4 v = MODE == 1 ? __builtin_nontemporal_load(p) : *p;
-            tile[row * SPLIT_LDS_STRIDE + 2 * cp] = make_float2(v[0], v[1]);
-            tile[row * SPLIT_LDS_STRIDE + 2 * cp + 1] = make_float2(v[2], v[3]);
-        }
-    }
-    __syncthreads();
-    const Dft16Split F = make_dft16_split(lane, sign);
-
-    // stage A: tile r0 = 4w+u, DFT over r1 (rows r0 + 16 r1), twiddle W_256^{r0 k0}
+        for (int r0 = 0; r0 < 16; ++r0)
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int r0 = 4 * w + u;
-        cf v[4], o[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float2 t = tile[(r0 + 16 * (4 * g + i)) * SPLIT_LDS_STRIDE + c16];
-            v[i] = cf{t.x, t.y};
-        }
-        dft16_split(v, F, o);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int k0 = 4 * g + i;
-            const cf x = cmul(o[i], twiddle(w4096, ((r0 * k0) & 255) * 16, sg));
-            tile[(r0 + 16 * k0) * SPLIT_LDS_STRIDE + c16] = make_float2(x.x, x.y);
-        }
-    }
-    __syncthreads();
-    // stage B: tile k0 = 4w+u, DFT over r0 (rows 16 k0 + r0) -> X[k0 + 16 k1]
-    cf res[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const int k0 = 4 * w + u;
-        cf v[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const float2 t = tile[(16 * k0 + 4 * g + i) * SPLIT_LDS_STRIDE + c16];
-            v[i] = cf{t.x, t.y};
-        }
-        dft16_split(v, F, res[u]);
-    }
-    __syncthreads();
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-        const u32 k0 = 4u * w + u;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const u32 k = k0 + 16u * (4u * g + i);
-            cf x = res[u][i];
-            if constexpr (MODE == 1) {
-                const u32 c = 16u * q + c16;
-                const u32 m = (c * k) & (N - 1u);
-                x = cmul(x, cmul(twiddle(wn_hi, m >> 8, sg), twiddle(wn_lo, m & 255u, sg)));
+            for (int j = 0; j < 4; ++j) {
+                const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(p + 256u * (r0 + 16u * j)));
+                dr[r0][j] = v[0];
+                di[r0][j] = v[1];
             }
-            tile[k * SPLIT_LDS_STRIDE + c16] = make_float2(x.x, x.y);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float m = 0.f;
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m = fmaxf(m, fmaxf(fabsf(dr[t][j]), fabsf(di[t][j])));
+    float s, inv;
+    wave_scale(m, s, inv);
+    const Dft16Split F = make_dft16_split(lane, SIGN);
+
+    // Stage 1 (tile r0; rows (g,i) = k0) and its epilogue factor
+    // W_256^{r0 k0} / 16, the 1/16 keeping the stage-2 inputs within the same
+    // bound (a DFT-16 grows magnitudes at most 16-fold).
+    cf step[4], rec[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        step[i] = cf{t_step[i].x, sg * t_step[i].y};
+        rec[i] = cf{0.0625f, 0.f};
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        f32x4 ore, oim;
+        dft16_split_b<true>(dr[t], di[t], s, F, ore, oim);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const cf v = cmul(cf{ore[i], oim[i]}, rec[i]);
+            if (t < 15) rec[i] = cmul(rec[i], step[i]);
+            dr[t][i] = v.x;
+            di[t][i] = v.y;
         }
     }
-    __syncthreads();
-    if constexpr (MODE == 1) {
-        // out row c = 16q + cl holds Y'_c[k], k = 0..255 contiguous
+    // (g,i) = k0, tile = r0  ->  (g,i) = r0, tile = k0
+    {
+        float tr[16][4], ti[16][4];
+        transpose_gi_tile(dr, tr);
+        transpose_gi_tile(di, ti);
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int idx = tid + 256 * j;
-            const int cl = idx >> 7, kp = idx & 127;
-            const float2 a = tile[(2 * kp) * SPLIT_LDS_STRIDE + cl];
-            const float2 bb = tile[(2 * kp + 1) * SPLIT_LDS_STRIDE + cl];
-            f32x4* p = reinterpret_cast<f32x4*>(dst + sbase + (size_t)(16u * q + cl) * 256 + 2 * kp);
-            const f32x4 v = {a.x, a.y, bb.x, bb.y};
-            __builtin_nontemporal_store(v, p);
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                dr[t][i] = tr[t][i];
+                di[t][i] = ti[t][i];
+            }
+    }
+    // Stage 2 (tile k0; rows (g,i) = k1), already in scaled units; inv2 undoes
+    // the scale s and the 1/16.
+    const float inv2 = inv * 16.f;
+    if constexpr (PASS == 2) {
+        // X[k + 256 m], m = k0 + 16 k1: column col, row m; each tile is stored
+        // as soon as it is transformed
+        float2* q = dst + sbase + col + 16384u * g;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            f32x4 ore, oim;
+            dft16_split_b<false>(dr[t], di[t], 1.f, F, ore, oim);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const f32x2 v = {ore[i] * inv2, oim[i] * inv2};
+                __builtin_nontemporal_store(v, reinterpret_cast<f32x2*>(q + 256u * (t + 16u * i)));
+            }
         }
     } else {
-        float2* out = dst + sbase + 16u * q;
 #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-            const int row = (tid >> 3) + 32 * j, cp = tid & 7;
-            const float2 a = tile[row * SPLIT_LDS_STRIDE + 2 * cp];
-            const float2 bb = tile[row * SPLIT_LDS_STRIDE + 2 * cp + 1];
-            const f32x4 v = {a.x, a.y, bb.x, bb.y};
-            __builtin_nontemporal_store(v, reinterpret_cast<f32x4*>(out + (size_t)row * 256 + 2 * cp));
+        for (int t = 0; t < 16; ++t) {
+            f32x4 ore, oim;
+            dft16_split_b<false>(dr[t], di[t], 1.f, F, ore, oim);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                dr[t][i] = ore[i];
+                di[t][i] = oim[i];
+            }
+        }
+        // Y_c[k] W_N^{c k}, k = (4g + i) + 16 t: W_N^{c(4g+i)} * (W_N^{16c})^t
+        // (two-level table), with the inverse scale; loaded here, after the
+        // data, so that they hold no registers during the two stages
+        float2 t_a[4][2], t_b[2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const u32 m = (col * (4u * g + i)) & 65535u;
+            t_a[i][0] = wn_hi[m >> 8];
+            t_a[i][1] = wn_lo[m & 255u];
+        }
+        {
+            const u32 m = (16u * col) & 65535u;
+            t_b[0] = wn_hi[m >> 8];
+            t_b[1] = wn_lo[m & 255u];
+        }
+        // (g,i) = k1, tile = k0 -> (g,i) = k0, tile = k1: four consecutive k
+        // per lane, stored transposed as T[k + 256 c]
+        float ta[16][4], tb[16][4];
+        transpose_gi_tile(dr, ta);
+        transpose_gi_tile(di, tb);
+        cf fa[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const cf v = cmul(cf{t_a[i][0].x, sg * t_a[i][0].y}, cf{t_a[i][1].x, sg * t_a[i][1].y});
+            fa[i] = cf{v.x * inv2, v.y * inv2};
+        }
+        const cf fstep = cmul(cf{t_b[0].x, sg * t_b[0].y}, cf{t_b[1].x, sg * t_b[1].y});
+        cf fb = {1.f, 0.f};
+        float2* q = dst + sbase + 256u * col + 4u * g;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            float o[8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                cf v = {ta[t][i], tb[t][i]};
+                v = cmul(v, t ? cmul(fa[i], fb) : fa[i]);
+                o[2 * i] = v.x;
+                o[2 * i + 1] = v.y;
+            }
+            if (t < 15) fb = t ? cmul(fb, fstep) : fstep;
+            const f32x4 o0 = {o[0], o[1], o[2], o[3]}, o1 = {o[4], o[5], o[6], o[7]};
+            __builtin_nontemporal_store(o0, reinterpret_cast<f32x4*>(q + 16u * t));
+            __builtin_nontemporal_store(o1, reinterpret_cast<f32x4*>(q + 16u * t + 2u));
         }
     }
 }
@@ -195,9 +251,9 @@ static hipError_t launch_split_t(const float2* in, float2* out, u32 n_signals, u
     for (u32 s0 = 0; s0 < n_signals; s0 += chunk) {
         const u32 cnt = min(chunk, n_signals - s0);
         const size_t off = (size_t)s0 * 65536;
-        hipLaunchKernelGGL((k_split256<1, SIGN>), dim3(cnt * 16), dim3(256), 0, st, in + off, out + off, w4096, wn_hi,
+        hipLaunchKernelGGL((k_split_col<1, SIGN>), dim3(cnt * 4), dim3(256), 0, st, in + off, out + off, w4096, wn_hi,
                            wn_lo);
-        hipLaunchKernelGGL((k_split256<2, SIGN>), dim3(cnt * 16), dim3(256), 0, st, out + off, out + off, w4096, wn_hi,
+        hipLaunchKernelGGL((k_split_col<2, SIGN>), dim3(cnt * 4), dim3(256), 0, st, out + off, out + off, w4096, wn_hi,
                            wn_lo);
     }
     return hipGetLastError();
