@@ -176,27 +176,56 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
     const u32 blk = wave / col_groups;
     const u32 col0 = (wave - blk * col_groups) * 16u;
     const float sg = sign < 0 ? 1.f : -1.f;
-    const Dft16Op op = make_dft16(lane, sign);
     constexpr u32 R = 16 * T;
     const size_t blk_base = (size_t)blk * R * s + col0 + c16;
 
+    // Table values before the data (global loads complete in issue order):
+    //   step[i] = W_{16T}^{k0}, k0 = 4g + i: the radix-T (T < 16) or
+    //   stage-1 (T = 16) twiddle W_{16T}^{k0*t} is its t-th power;
+    //   epilogue: output row k0 + 16*t (T < 16), t + 16*k0 (T = 16) or k0
+    //   (T = 1) gets F0[i] * S^t with F0[i] = W_M^{(xc + k0*sa*u)*rj}
+    //   (u = 16 for T = 16, else 1) and S = W_M^{sa*rj*(T == 16 ? 1 : 16)}.
+    float2 t_step[4];
+    if constexpr (T > 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t_step[i] = w4096[(4 * g + i) * (256 / T)];
+    }
+    struct RawW {
+        float2 h, l;
+    };
+    RawW r_f0[4], r_s;
+    if constexpr (EPI) {
+        const u32 rj = epi_rev(epi, blk);
+        const u32 sa = s >> epi.shift, xc = (col0 + (u32)c16) >> epi.shift;
+        const u32 u = T == 16 ? 16u : 1u;
+        auto ld = [&](u32 m) -> RawW {
+            m &= epi.m_mask;
+            return RawW{epi.hi[m >> epi.lo_bits], epi.lo[m & ((1u << epi.lo_bits) - 1u)]};
+        };
+#pragma unroll
+        for (int i = 0; i < 4; ++i) r_f0[i] = ld((xc + (4u * g + i) * u * sa) * rj);
+        r_s = ld(sa * rj * (T == 16 ? 1u : 16u));
+    }
+    __builtin_amdgcn_sched_barrier(0);
     u32 d[T][4];
 #pragma unroll
     for (int t = 0; t < T; ++t)
 #pragma unroll
         for (int i = 0; i < 4; ++i) d[t][i] = buf[blk_base + (size_t)(16 * t + 4 * g + i) * s];
+    __builtin_amdgcn_sched_barrier(0);
 
-    // Epilogue: output row k0 + 16*t (T < 16), t + 16*k0 (T = 16) or k0
-    // (T = 1), k0 = 4g + i, gets F0[i] * S^t with F0[i] = W_M^{(xc + k0*sa*u)*rj}
-    // (u = 16 for T = 16, else 1) and S = W_M^{sa*rj*(T == 16 ? 1 : 16)}.
+    const Dft16Op op = make_dft16(lane, sign);
+    cf step[4];
+    if constexpr (T > 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) step[i] = cf{t_step[i].x, sg * t_step[i].y};
+    }
+    auto rawv = [&](const RawW& r) -> cf { return cmul(cf{r.h.x, sg * r.h.y}, cf{r.l.x, sg * r.l.y}); };
     cf F0[4], S = {1.f, 0.f};
     if constexpr (EPI) {
-        const u32 rj = epi_rev(epi, blk);
-        const u32 sa = s >> epi.shift, xc = (col0 + (u32)c16) >> epi.shift;
-        const u32 u = T == 16 ? 16u : 1u;
 #pragma unroll
-        for (int i = 0; i < 4; ++i) F0[i] = epi_w(epi, (xc + (4u * g + i) * u * sa) * rj, sg);
-        if constexpr (T > 1) S = epi_w(epi, sa * rj * (T == 16 ? 1u : 16u), sg);
+        for (int i = 0; i < 4; ++i) F0[i] = rawv(r_f0[i]);
+        if constexpr (T > 1) S = rawv(r_s);
     }
 
     if constexpr (T == 1) {
@@ -210,7 +239,6 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
             buf[blk_base + (size_t)row * s] = pack(v);
         }
     } else if constexpr (T < 16) {
-        constexpr int STRIDE = 256 / T;  // W_{16T}^m = W_4096^{m*STRIDE}
         cf acc[T][4];
 #pragma unroll
         for (int t = 0; t < T; ++t) {
@@ -224,8 +252,12 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
             const u32 k0 = 4u * g + i;
             cf v[T];
             v[0] = acc[0][i];
+            cf p = step[i];  // W_{16T}^{k0*t}
 #pragma unroll
-            for (int t = 1; t < T; ++t) v[t] = cmul(acc[t][i], twiddle(w4096, ((k0 * t) * STRIDE) & 4095u, sg));
+            for (int t = 1; t < T; ++t) {
+                v[t] = cmul(acc[t][i], p);
+                if (t + 1 < T) p = cmul(p, step[i]);
+            }
             small_dft<T>(v, sign);
             cf f = EPI ? F0[i] : cf{1.f, 0.f};
 #pragma unroll
@@ -240,8 +272,12 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
             }
         }
     } else {
-        // T == 16: stage 0 -> (g,i) = k0, tile = j1; twiddle W_256^{k0*j1};
-        // transpose to (g,i) = j1, tile = k0; stage 1 -> (g,i) = k1.
+        // T == 16: stage 0 -> (g,i) = k0, tile = j1; twiddle W_256^{k0*j1}
+        // (recurrence over the tiles); transpose to (g,i) = j1, tile = k0;
+        // stage 1 -> (g,i) = k1.
+        cf rec[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rec[i] = step[i];
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             f32x4 re, im;
@@ -249,26 +285,15 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 cf v = {re[i], im[i]};
-                const u32 k0 = 4u * g + i;
-                if (t) v = cmul(v, twiddle(w4096, ((k0 * t) & 255u) * 16u, sg));
+                if (t) {
+                    v = cmul(v, rec[i]);
+                    if (t < 15) rec[i] = cmul(rec[i], step[i]);
+                }
                 d[t][i] = pack(v);
             }
         }
         u32 e[16][4];
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) e[(t & 12) | i][t & 3] = d[t][i];
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (!(t & 4))
-#pragma unroll
-                for (int i = 0; i < 4; ++i) swap_l4(e[t][i], e[t | 4][i]);
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if (!(t & 8))
-#pragma unroll
-                for (int i = 0; i < 4; ++i) swap_l5(e[t][i], e[t | 8][i]);
+        transpose_gi_tile(d, e);
         // now (g,i) = j1, tile = k0
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
@@ -286,6 +311,79 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
             }
         }
     }
+}
+
+// ----------------------------------------------------------------------------
+// k_cols_q<T>: the strided radix-16T pass (T = 1, 2, 4) with four adjacent
+// columns per lane (16-byte loads and stores, 64 columns per wave, 4x the
+// bytes in flight of k_cols), for passes without an epilogue twiddle.
+// Lane (c16, g) owns columns col0 + 4*c16 + q, q = 0..3; MFMA q takes
+// column 4*c16 + q as its column c16.
+template <int T>
+__global__ __launch_bounds__(256) void k_cols_q(u32* __restrict__ buf, u32 s_log2, u32 n_waves, int sign,
+                                                const float2* __restrict__ w4096) {
+    static_assert(T == 1 || T == 2 || T == 4, "k_cols_q: T = 1, 2, 4");
+    const int lane = threadIdx.x & 63;
+    const int c16 = lane & 15, g = lane >> 4;
+    const u32 wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave >= n_waves) return;
+    const u32 s = 1u << s_log2;
+    const u32 col_groups = s >> 6;
+    const u32 blk = wave / col_groups;
+    const u32 col0 = (wave - blk * col_groups) * 64u;
+    constexpr u32 R = 16 * T;
+    const float sg = sign < 0 ? 1.f : -1.f;
+    // W_{16T}^{k0*t}, k0 = 4g + i, t >= 1: loaded before the data
+    float2 tw[4][T > 1 ? T - 1 : 1];
+    if constexpr (T > 1) {
+        constexpr int STRIDE = 256 / T;  // W_{16T}^m = W_4096^{m*STRIDE}
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int t = 1; t < T; ++t) tw[i][t - 1] = w4096[(((4 * g + i) * t) * STRIDE) & 4095];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    u32* const base = buf + (size_t)blk * R * s + col0 + 4u * c16;
+    u32 d[T][4][4];  // [tile][row word i][column q]
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) load4_nt(d[t][i], base + (size_t)(16 * t + 4 * g + i) * s);
+    __builtin_amdgcn_sched_barrier(0);
+    const Dft16Op op = make_dft16(lane, sign);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        cf acc[T][4];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            const u32 a[4] = {d[t][0][q], d[t][1][q], d[t][2][q], d[t][3][q]};
+            f32x4 re, im;
+            dft16_b(a, op, re, im);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) acc[t][i] = cf{re[i], im[i]};
+        }
+        if constexpr (T > 1) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                cf v[T];
+                v[0] = acc[0][i];
+#pragma unroll
+                for (int t = 1; t < T; ++t) v[t] = cmul(acc[t][i], cf{tw[i][t - 1].x, sg * tw[i][t - 1].y});
+                small_dft<T>(v, sign);
+#pragma unroll
+                for (int t = 0; t < T; ++t) acc[t][i] = v[t];
+            }
+        }
+        // output row k0 + 16t (k0 = 4g + i) of column q
+#pragma unroll
+        for (int t = 0; t < T; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) d[t][i][q] = pack(acc[t][i]);
+    }
+#pragma unroll
+    for (int t = 0; t < T; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) store4_nt(base + (size_t)(4 * g + i + 16 * t) * s, d[t][i]);
 }
 
 // ----------------------------------------------------------------------------
@@ -377,6 +475,13 @@ hipError_t launch_rows(int T, u32* buf, u32 n_tiles, int sign, const float2* w40
 template <int T>
 static hipError_t launch_cols_t(u32* buf, u32 s_log2, u32 outer, int sign, const float2* w4096,
                                 const EpiParams* epi, hipStream_t st) {
+    if constexpr (T <= 4) {
+        if (!epi && s_log2 >= 6) {
+            const u32 waves = outer * ((1u << s_log2) >> 6);
+            hipLaunchKernelGGL((k_cols_q<T>), dim3((waves + 3) / 4), dim3(256), 0, st, buf, s_log2, waves, sign, w4096);
+            return hipGetLastError();
+        }
+    }
     const u32 waves = outer * ((1u << s_log2) >> 4);
     const u32 blocks = (waves + 3) / 4;
     if (epi) hipLaunchKernelGGL((k_cols<T, true>), dim3(blocks), dim3(256), 0, st, buf, s_log2, waves, sign, w4096, *epi);

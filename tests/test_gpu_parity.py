@@ -105,7 +105,8 @@ def test_multipass_forward_vs_oracle(n, batch):
 
 
 @pytest.mark.parametrize("n,batch,direction", [(1 << 19, 1, tcfft_amd.FORWARD), (1 << 20, 2, tcfft_amd.FORWARD),
-                                               (1 << 20, 1, tcfft_amd.INVERSE), (1 << 22, 1, tcfft_amd.FORWARD),
+                                               (1 << 20, 1, tcfft_amd.INVERSE), (1 << 21, 1, tcfft_amd.FORWARD),
+                                               (1 << 22, 1, tcfft_amd.FORWARD),
                                                (1 << 24, 1, tcfft_amd.FORWARD), (1 << 25, 1, tcfft_amd.INVERSE)])
 def test_large_vs_numpy(n, batch, direction):
     """Sizes above 2^18 (every radix of the pass after the 65536-point
