@@ -21,7 +21,8 @@ Prints one JSON line:
                (the single-precision host FFT the GPU path replaces)
   configs    = the other workloads (N=4096 x 1024; N=2^20 x 16; 2D 1024^2 x 8;
                split precision; uneven batch; FFT-conv layer), each with its
-               time, GFLOP/s, HBM fraction and rel-L2 vs FP64
+               time, GFLOP/s, HBM fraction and rel-L2 vs FP64, and hipFFT's
+               half-precision transform on three of them as the vendor line
 """
 from __future__ import annotations
 
@@ -341,6 +342,50 @@ def bench_conv(torch, reps):
             "note": "forward = 2 FFTs + pointwise product; forward+backward = 6 FFTs + products (torch autograd)"}
 
 
+def hipfft_fp16(torch, dims, batch, reps):
+    """The vendor library on the same workload: hipFFT (rocFFT) half-precision
+    C2C through hipfftXtMakePlanMany(HIP_C_16F), in place, natural order,
+    timed with events over `reps` eager execs after a warm-up."""
+    import ctypes
+    try:
+        lib = ctypes.CDLL("/opt/rocm/lib/libhipfft.so")
+    except OSError as e:
+        return {"error": f"libhipfft not loadable: {e}"}
+    HIP_C_16F = 6
+    h = ctypes.c_void_p()
+    if lib.hipfftCreate(ctypes.byref(h)) != 0:
+        return {"error": "hipfftCreate failed"}
+    rank = len(dims)
+    n = (ctypes.c_longlong * rank)(*dims)
+    total = int(np.prod(dims))
+    ws = ctypes.c_size_t()
+    rc = lib.hipfftXtMakePlanMany(h, rank, n, None, ctypes.c_longlong(1), ctypes.c_longlong(total), HIP_C_16F, None,
+                                  ctypes.c_longlong(1), ctypes.c_longlong(total), HIP_C_16F, ctypes.c_longlong(batch),
+                                  ctypes.byref(ws), HIP_C_16F)
+    if rc != 0:
+        lib.hipfftDestroy(h)
+        return {"error": f"hipfftXtMakePlanMany returned {rc}"}
+    stream = torch.cuda.current_stream().cuda_stream
+    lib.hipfftSetStream(h, ctypes.c_void_p(stream))
+    rng = np.random.default_rng(2)
+    x = fp16_uniform(rng, (batch,) + tuple(dims))
+    bufs = rotation(torch, x.shape, torch.float16, torch.from_numpy(x))
+    lib.hipfftXtExec(h, ctypes.c_void_p(bufs[0].data_ptr()), ctypes.c_void_p(bufs[0].data_ptr()), -1)
+    torch.cuda.synchronize()
+    k = min(batch, 2)
+    xs = x[:k, ..., 0].astype(np.float64) + 1j * x[:k, ..., 1]
+    ref = np.fft.fftn(xs, axes=tuple(range(1, rank + 1)))
+    err = rel(to_c(bufs[0][:k]), ref)
+    nb = len(bufs)
+    ms = eager_time(torch, lambda i: lib.hipfftXtExec(h, ctypes.c_void_p(bufs[i % nb].data_ptr()),
+                                                       ctypes.c_void_p(bufs[i % nb].data_ptr()), -1), reps)
+    lib.hipfftDestroy(h)
+    out = summary(ms, total, batch, 8.0 * total * batch)
+    out.update({"rel_l2_vs_fp64": err, "library": "hipFFT/rocFFT (/opt/rocm), HIP_C_16F, in place, eager",
+                "work_bytes": int(ws.value)})
+    return out
+
+
 def stream_copy(torch):
     """Device-to-device copy bandwidth over a > MALL rotation (TB/s, read+write)."""
     src = rotation(torch, (64 << 20,), torch.float32, torch.zeros(64 << 20))  # 256 MiB each
@@ -458,6 +503,9 @@ def main():
         cfg["c3_split_n2^16_b256"] = bench_split(torch, tcfft_amd, reps, ms_plain=ms)
         cfg["c4_uneven_4096_signals"] = bench_uneven(torch, tcfft_amd, reps)
         cfg["c5_fftconv_64x256x8192"] = bench_conv(torch, 5)
+        cfg["hipfft_fp16_n2^16_b256"] = hipfft_fp16(torch, [n], batch, reps)
+        cfg["hipfft_fp16_n4096_b1024"] = hipfft_fp16(torch, [4096], 1024, reps)
+        cfg["hipfft_fp16_2d_1024x1024_b8"] = hipfft_fp16(torch, [1024, 1024], 8, reps)
         cfg["stream_copy_tbs"] = stream_copy(torch)
         out["configs"] = cfg
     if not args.no_cpu:
