@@ -123,7 +123,7 @@ class _FusedConv(torch.autograd.Function):
     gH = (1/n) sum_b conj(F x_b) * F gy_b from the spectra both passes store."""
 
     @staticmethod
-    def forward(ctx, x, weight):
+    def forward(ctx, x, weight, want_filter_grad: bool):
         _check(x)
         x = x.contiguous()
         n, C = x.shape[-2], weight.shape[0]
@@ -134,7 +134,8 @@ class _FusedConv(torch.autograd.Function):
         stream = torch.cuda.current_stream(x.device).cuda_stream
         y = torch.empty_like(x)
         # the spectrum of x is kept only when the filter gradient will be needed
-        spec = torch.empty_like(x) if ctx.needs_input_grad[1] else None
+        # (grad mode is off inside forward, so the caller passes it in)
+        spec = torch.empty_like(x) if want_filter_grad else None
         plan.conv_exec(C, x.data_ptr(), h_rev.data_ptr(), y.data_ptr(), False,
                        spec.data_ptr() if spec is not None else 0, stream)
         ctx.save_for_backward(h_rev, spec if spec is not None else torch.empty(0, device=x.device))
@@ -157,7 +158,7 @@ class _FusedConv(torch.autograd.Function):
             gh_rev = torch.empty(C, n, 2, dtype=torch.float32, device=gy.device)
             plan.conv_filter_grad(C, spec.data_ptr(), gspec.data_ptr(), gh_rev.data_ptr(), stream)
             gw = gh_rev.index_select(1, _rev(n, gy.device)[1])
-        return gx, gw
+        return gx, gw, None
 
 
 class FFTConv1d(torch.nn.Module):
@@ -176,7 +177,7 @@ class FFTConv1d(torch.nn.Module):
 
     def forward(self, x: torch.Tensor) -> torch.Tensor:
         if self.n in FUSED_N:
-            return _FusedConv.apply(x, self.weight)
+            return _FusedConv.apply(x, self.weight, torch.is_grad_enabled() and self.weight.requires_grad)
         X = fft(x)
         Y = complex_mul(X, self.weight)
         y = fft_adjoint(Y)
