@@ -40,9 +40,11 @@ EXPORTS = (
     "tcfft_exec_split", "tcfft_plan_set_option",
     "tcfft_multi_create", "tcfft_multi_destroy", "tcfft_multi_info", "tcfft_multi_exec",
     "tcfft_multi_set_schedule", "tcfft_multi_autotune",
-    "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad",
+    "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad", "tcfft_plan_autotune",
 )
 OPT_SPLIT_CHUNK = 1
+OPT_BIG_DELAY = 2
+OPT_BIG_GROUPS = 3
 
 
 class TcfftError(RuntimeError):
@@ -84,6 +86,7 @@ def lib():
         L.tcfft_exec_passes.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         L.tcfft_exec_split.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_plan_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+        L.tcfft_plan_autotune.argtypes = [_vp, _vp, _vp, _i32p, _i32p, ctypes.POINTER(ctypes.c_float)]
         L.tcfft_exec_dif.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_conv_exec.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
         L.tcfft_conv_filter_grad.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
@@ -193,6 +196,15 @@ class _Plan:
 
     def set_option(self, option: int, value: int):
         _check(lib().tcfft_plan_set_option(self._h, option, value), "tcfft_plan_set_option")
+
+    def autotune(self, data_ptr: int, stream: int = 0):
+        """Picks the fastest start schedule of the single-pass kernel by timing
+        forward transforms of the buffer at data_ptr (overwritten).  Returns
+        (groups, delay, ms)."""
+        g, d, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
+        _check(lib().tcfft_plan_autotune(self._h, _vp(data_ptr), _vp(stream), ctypes.byref(g), ctypes.byref(d),
+                                         ctypes.byref(ms)), "tcfft_plan_autotune")
+        return g.value, d.value, ms.value
 
     def destroy(self):
         if self._h.value:

@@ -436,8 +436,12 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
 template <int W, bool EPI, int SIGN>
 __global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, const float2* __restrict__ w4096,
                                                 const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
-                                                u32 wn_lo_bits, EpiParams epi) {
+                                                u32 wn_lo_bits, EpiParams epi, BigSched sched) {
     extern __shared__ u32 lds[];
+    if (sched.delay && blockIdx.x < sched.resident) {
+        const u32 quanta = ((blockIdx.x >> 3) % sched.groups) * sched.delay;
+        for (u32 i = 0; i < quanta; ++i) __builtin_amdgcn_s_sleep(16);
+    }
     big_body<W, BigCfg<W>::ROUNDS, EPI, SIGN>(buf + (size_t)blockIdx.x * BigCfg<W>::N, threadIdx.x >> 6, lds,
                                              blockIdx.x, w4096, wn_hi, wn_lo, wn_lo_bits, epi);
 }
@@ -525,10 +529,27 @@ static hipError_t set_lds_once(K kernel, size_t bytes) {
     return status;
 }
 
+// Workgroups of kernel k resident at once on the device (cached per kernel).
+template <class K>
+static u32 resident_blocks(K kernel, int threads, size_t lds) {
+    static std::once_flag flag;
+    static u32 blocks = 0;
+    std::call_once(flag, [&] {
+        int dev = 0, cus = 0, per_cu = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kernel, threads, lds) == hipSuccess)
+            blocks = (u32)(cus * per_cu);
+    });
+    return blocks;
+}
+
 template <int W, int SIGN>
 static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, const float2* wn_hi,
-                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, hipStream_t st) {
+                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, BigSched sched,
+                               hipStream_t st) {
     const size_t lds = BigCfg<W>::LDS_WORDS * sizeof(u32);
+    if (sched.groups == 0) sched.delay = 0;
     if (epi) {
         // the folded epilogue covers a first pass of 65536 contiguous points
         // followed by a pass at step 65536 along the same axis
@@ -537,7 +558,9 @@ static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, con
             auto k = k_big<W, true, SIGN>;
             hipError_t e = set_lds_once(k, lds);
             if (e != hipSuccess) return e;
-            hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits, *epi);
+            if (sched.delay) sched.resident = resident_blocks(k, 64 * W, lds);
+            hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits, *epi,
+                               sched);
         } else {
             return hipErrorInvalidValue;
         }
@@ -545,26 +568,28 @@ static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, con
         auto k = k_big<W, false, SIGN>;
         hipError_t e = set_lds_once(k, lds);
         if (e != hipSuccess) return e;
+        if (sched.delay) sched.resident = resident_blocks(k, 64 * W, lds);
         hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits,
-                           EpiParams{});
+                           EpiParams{}, sched);
     }
     return hipGetLastError();
 }
 
 template <int W>
 static hipError_t launch_big_t(u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
-                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, hipStream_t st) {
-    return sign < 0 ? launch_big_t<W, -1>(buf, n_signals, w4096, wn_hi, wn_lo, lo_bits, epi, st)
-                    : launch_big_t<W, +1>(buf, n_signals, w4096, wn_hi, wn_lo, lo_bits, epi, st);
+                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, BigSched sched,
+                               hipStream_t st) {
+    return sign < 0 ? launch_big_t<W, -1>(buf, n_signals, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st)
+                    : launch_big_t<W, +1>(buf, n_signals, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st);
 }
 
 hipError_t launch_big(int W, u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
-                      const float2* wn_lo, u32 lo_bits, const EpiParams* epi, hipStream_t st) {
+                      const float2* wn_lo, u32 lo_bits, const EpiParams* epi, BigSched sched, hipStream_t st) {
     switch (W) {
-    case 2: return launch_big_t<2>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
-    case 4: return launch_big_t<4>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
-    case 8: return launch_big_t<8>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
-    case 16: return launch_big_t<16>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, st);
+    case 2: return launch_big_t<2>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st);
+    case 4: return launch_big_t<4>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st);
+    case 8: return launch_big_t<8>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st);
+    case 16: return launch_big_t<16>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st);
     default: return hipErrorInvalidValue;
     }
 }

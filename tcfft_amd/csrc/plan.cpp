@@ -184,6 +184,8 @@ struct tcfft_plan_s {
     size_t workspace_bytes = 0;
     size_t total = 0;  // elements in the buffer
     int split_chunk = 128;  // signals per pass pair in tcfft_exec_split
+    int big_delay = -1;   // start schedule of the single-pass kernel: -1 = per-width default
+    int big_groups = 3;
 };
 
 namespace {
@@ -279,6 +281,14 @@ int create_common(tcfft_plan* plan, const Layout& lay) {
 
 bool valid(tcfft_plan p) { return p && p->magic == kMagic; }
 
+// Start schedule of the single-pass kernel of width W.  Defaults measured on
+// MI355X (3 groups; 2^15 x 512: 26.2 -> 24.0 us with delay 1, 2^16 x 256:
+// 28.9 -> 26.5 us with delay 2, 2^14 x 2048: 53.9 -> 50.4 us with delay 1).
+tcfft::BigSched big_sched(const tcfft_plan_s* p, int W) {
+    const int delay = p->big_delay >= 0 ? p->big_delay : (W == 4 || W == 8 ? 1 : 2);
+    return tcfft::BigSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
+}
+
 int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t first = 0, size_t count = ~(size_t)0) {
     const size_t end = count > p->lay.passes.size() - first ? p->lay.passes.size() : first + count;
     for (size_t i = first; i < end; ++i) {
@@ -288,7 +298,7 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
         if (ps.kind == 0) e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, st);
         else if (ps.kind == 2)
             e = tcfft::launch_big(ps.radix, buf, ps.n_tiles, sign, p->w4096, ps.big_hi, ps.big_lo, ps.big_M_log2 / 2,
-                                  epi, st);
+                                  epi, big_sched(p, ps.radix), st);
         else e = tcfft::launch_cols(ps.radix, buf, ps.s_log2, ps.outer, sign, p->w4096, epi, st);
         if (e != hipSuccess) return hip_fail(e);
     }
@@ -512,8 +522,80 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
         if (value < 0) return TCFFT_INVALID_VALUE;
         plan->split_chunk = value;
         return TCFFT_SUCCESS;
+    case TCFFT_OPT_BIG_DELAY:
+        if (value < -1 || value > 64) return TCFFT_INVALID_VALUE;
+        plan->big_delay = value;
+        return TCFFT_SUCCESS;
+    case TCFFT_OPT_BIG_GROUPS:
+        if (value < 1 || value > 8) return TCFFT_INVALID_VALUE;
+        plan->big_groups = value;
+        return TCFFT_SUCCESS;
     default: return TCFFT_INVALID_VALUE;
     }
+}
+
+int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_groups, int* best_delay,
+                        float* best_ms) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
+    bool has_big = false;
+    for (const Pass& ps : plan->lay.passes) has_big |= ps.kind == 2;
+    DeviceGuard guard(plan->device);
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t a, b;
+    if (hipEventCreate(&a) != hipSuccess) return TCFFT_HIP_ERROR;
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return TCFFT_HIP_ERROR;
+    }
+    struct Cand {
+        int groups, delay;
+        float ms;
+    };
+    std::vector<Cand> cands;
+    if (has_big) {
+        const int g3[] = {0, 1, 2, 3};
+        for (int d : g3) cands.push_back({3, d, 1e30f});
+        cands.push_back({5, 1, 1e30f});
+        cands.push_back({5, 2, 1e30f});
+        cands.push_back({7, 1, 1e30f});
+    } else {
+        cands.push_back({plan->big_groups, plan->big_delay, 1e30f});
+    }
+    const int saved_groups = plan->big_groups, saved_delay = plan->big_delay;
+    const int reps = 10;
+    int rc = TCFFT_SUCCESS;
+    // two interleaved rounds, the faster of each candidate's two timings kept
+    for (int round = 0; round < 2 && !rc; ++round) {
+        for (Cand& c : cands) {
+            plan->big_groups = c.groups;
+            plan->big_delay = c.delay;
+            for (int i = 0; i < 2 && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
+            if (rc) break;
+            if (hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            for (int i = 0; i < reps && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
+            if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            float ms = 0.f;
+            if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            if (rc) break;
+            c.ms = std::min(c.ms, ms / reps);
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    plan->big_groups = saved_groups;
+    plan->big_delay = saved_delay;
+    if (rc) return rc;
+    const Cand* best = &cands[0];
+    for (const Cand& c : cands)
+        if (c.ms < best->ms) best = &c;
+    plan->big_groups = best->groups;
+    plan->big_delay = best->delay;
+    if (best_groups) *best_groups = best->groups;
+    if (best_delay) *best_delay = best->delay;
+    if (best_ms) *best_ms = best->ms;
+    return TCFFT_SUCCESS;
 }
 
 int tcfft_plan_num_passes(tcfft_plan plan, int* n_passes) {

@@ -1,0 +1,92 @@
+"""Start schedule of the single-pass kernel (plan options TCFFT_OPT_BIG_DELAY /
+TCFFT_OPT_BIG_GROUPS and tcfft_plan_autotune): every schedule gives output
+bit-identical to the unstaggered launch, the default and tuned schedules match
+FP64 numpy (rel-L2 <= 1e-3 up to 2^16, 1.5e-3 above), and bad option values
+are rejected."""
+import sys
+
+import numpy as np
+import pytest
+
+import tcfft_amd
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def note(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def dropin_input(n, batch, seed):
+    rng = np.random.default_rng(seed)
+    x = (rng.uniform(-1, 1, (batch, n)).astype(np.float16).astype(np.float64)
+         + 1j * rng.uniform(-1, 1, (batch, n)).astype(np.float16).astype(np.float64))
+    rad, _ = tcfft_amd.query_1d(n, batch)
+    xr = x[:, tcfft_amd.gen_rev(n, rad)]
+    h = np.ascontiguousarray(np.stack([xr.real, xr.imag], -1).astype(np.float16))
+    return x, torch.from_numpy(h).cuda()
+
+
+def rel_l2(t, want):
+    h = t.cpu().numpy().astype(np.float64)
+    got = h[..., 0] + 1j * h[..., 1]
+    return float(np.linalg.norm(got - want) / np.linalg.norm(want))
+
+
+# sizes whose first pass is the single-pass kernel of each width, and a
+# longer transform whose first pass folds the next pass's twiddle
+@pytest.mark.parametrize("n,batch", [(8192, 300), (16384, 260), (32768, 270), (65536, 259), (1 << 20, 3)])
+def test_schedules_bit_identical(n, batch):
+    note(f"[gpu] schedule n={n} batch={batch}")
+    x, d = dropin_input(n, batch, n + batch)
+    plan = tcfft_amd.Plan1D(n, batch)
+    plan.set_option(tcfft_amd.OPT_BIG_DELAY, 0)
+    ref = d.clone()
+    plan.exec(ref.data_ptr())
+    torch.cuda.synchronize()
+    tol = 1e-3 if n <= 65536 else 1.5e-3
+    assert rel_l2(ref, np.fft.fft(x, axis=1)) <= tol
+    for groups, delay in [(3, -1), (3, 1), (3, 3), (5, 2), (7, 1), (2, 4), (1, 5), (8, 64)]:
+        plan.set_option(tcfft_amd.OPT_BIG_GROUPS, groups)
+        plan.set_option(tcfft_amd.OPT_BIG_DELAY, delay)
+        for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
+            t = d.clone()
+            plan.exec(t.data_ptr(), direction)
+            torch.cuda.synchronize()
+            if direction == tcfft_amd.FORWARD:
+                assert torch.equal(t, ref), (groups, delay)
+            else:
+                assert rel_l2(t, np.fft.ifft(x, axis=1) * n) <= tol
+    plan.destroy()
+
+
+@pytest.mark.parametrize("n,batch", [(65536, 256), (4096, 64)])
+def test_autotune_then_parity(n, batch):
+    note(f"[gpu] plan autotune n={n} batch={batch}")
+    x, d = dropin_input(n, batch, 11)
+    plan = tcfft_amd.Plan1D(n, batch)
+    scratch = d.clone()
+    groups, delay, ms = plan.autotune(scratch.data_ptr())
+    torch.cuda.synchronize()
+    assert ms > 0
+    if n == 65536:
+        assert (groups, delay) in {(3, 0), (3, 1), (3, 2), (3, 3), (5, 1), (5, 2), (7, 1)}
+    t = d.clone()
+    plan.exec(t.data_ptr())
+    torch.cuda.synchronize()
+    assert rel_l2(t, np.fft.fft(x, axis=1)) <= 1e-3
+    plan.destroy()
+
+
+def test_bad_schedule_options():
+    plan = tcfft_amd.Plan1D(65536, 1)
+    for opt, val in [(tcfft_amd.OPT_BIG_DELAY, -2), (tcfft_amd.OPT_BIG_DELAY, 65), (tcfft_amd.OPT_BIG_GROUPS, 0),
+                     (tcfft_amd.OPT_BIG_GROUPS, 9), (99, 1)]:
+        with pytest.raises(tcfft_amd.TcfftError):
+            plan.set_option(opt, val)
+    d = torch.zeros(65536, 2, dtype=torch.float16, device="cuda")
+    with pytest.raises(tcfft_amd.TcfftError):
+        plan.autotune(d.data_ptr() + 4)
+    plan.destroy()
