@@ -85,6 +85,14 @@ __device__ __forceinline__ float cos16_exact(int m) {
 }
 __device__ __forceinline__ float sin16_exact(int m) { return cos16_exact(m - 4); }
 
+// Register budget of the MFMA kernels launched with at most 256 threads.
+// Such a workgroup may run one wave per SIMD, so without a cap each lane may
+// use 512 registers (256 VGPRs + 256 AGPRs), and with that budget the
+// compiler writes every MFMA result to AGPRs and copies it out with one
+// v_accvgpr_read per register (4 VALU ops per MFMA).  At least 2 waves per
+// SIMD caps the budget at 256 and keeps MFMA results in VGPRs.
+#define TCFFT_MFMA_BUDGET __attribute__((amdgpu_waves_per_eu(2)))
+
 struct Dft16Op {
     f16x8 re, im;
 };

@@ -148,7 +148,7 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
 }
 
 template <int T, bool EPI, int SIGN>
-__global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles,
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows(u32* __restrict__ buf, u32 n_tiles,
                                               const float2* __restrict__ w4096, EpiParams epi) {
     const int lane = threadIdx.x & 63;
     const u32 wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void k_rows(u32* __restrict__ buf, u32 n_tiles
 // j0 (adjacent rows), stage 1 a radix-T over j1 with W_{16T}^{k0*j1}.
 // Lane (c16, g) word i of tile j1 holds row j0 = 4g+i, column c16.
 template <int T, bool EPI>
-__global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2, u32 n_waves, int sign,
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_cols(u32* __restrict__ buf, u32 s_log2, u32 n_waves, int sign,
                                               const float2* __restrict__ w4096, EpiParams epi) {
     const int lane = threadIdx.x & 63;
     const int c16 = lane & 15, g = lane >> 4;
@@ -320,7 +320,7 @@ __global__ __launch_bounds__(256) void k_cols(u32* __restrict__ buf, u32 s_log2,
 // Lane (c16, g) owns columns col0 + 4*c16 + q, q = 0..3; MFMA q takes
 // column 4*c16 + q as its column c16.
 template <int T>
-__global__ __launch_bounds__(256) void k_cols_q(u32* __restrict__ buf, u32 s_log2, u32 n_waves, int sign,
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_cols_q(u32* __restrict__ buf, u32 s_log2, u32 n_waves, int sign,
                                                 const float2* __restrict__ w4096) {
     static_assert(T == 1 || T == 2 || T == 4, "k_cols_q: T = 1, 2, 4");
     const int lane = threadIdx.x & 63;
@@ -434,7 +434,7 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
 }
 
 template <int W, bool EPI, int SIGN>
-__global__ __launch_bounds__(64 * W) void k_big(u32* __restrict__ buf, const float2* __restrict__ w4096,
+__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big(u32* __restrict__ buf, const float2* __restrict__ w4096,
                                                 const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
                                                 u32 wn_lo_bits, EpiParams epi, BigSched sched) {
     extern __shared__ u32 lds[];
