@@ -121,9 +121,9 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
 /* Plan options. TCFFT_OPT_SPLIT_CHUNK: signals per pass pair of
  * tcfft_exec_split (0 = whole batch; default 128), which bounds the
  * intermediate's footprint so the second pass reads it from cache.
- * TCFFT_OPT_BIG_DELAY (-1..64, default -1 = per-width default) and
- * TCFFT_OPT_BIG_GROUPS (1..8, default 3): start schedule of the single-pass
- * kernel (first passes of 8192..65536 points); the workgroups resident at
+ * TCFFT_OPT_BIG_DELAY (-1..64, default -1 = per-kernel default) and
+ * TCFFT_OPT_BIG_GROUPS (1..8, default 3): start schedule of the first pass
+ * (contiguous lines of up to 65536 points); the workgroups resident at
  * launch form GROUPS groups and group k starts k * DELAY quanta (1024 GPU
  * cycles each) late, so that one group's loads overlap another's stores
  * instead of all workgroups loading, computing and storing in lockstep.
@@ -133,8 +133,7 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
 /* Times the plan's forward transform under each start schedule candidate
  * (groups, delay) on `data` (which it overwrites with transforms of
- * itself), keeps the fastest in the plan and reports it.  Plans without a
- * single-pass kernel keep their schedule and report its time.  No reference
+ * itself), keeps the fastest in the plan and reports it.  No reference
  * counterpart (the reference has one fixed launch per size). */
 int tcfft_plan_autotune(tcfft_plan plan, void *data, void *stream, int *best_groups, int *best_delay,
                         float *best_ms);

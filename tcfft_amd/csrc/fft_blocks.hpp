@@ -8,6 +8,16 @@
 
 namespace tcfft {
 
+// Start delay of a first-pass workgroup under schedule s (see StartSched):
+// s.delay sleep quanta of 1024 cycles per group index, for the workgroups of
+// the first resident wave only.  The loop count is uniform and bounded.
+__device__ __forceinline__ void start_delay(const StartSched& s) {
+    if (s.delay && blockIdx.x < s.resident) {
+        const u32 quanta = ((blockIdx.x >> 3) % s.groups) * s.delay;
+        for (u32 i = 0; i < quanta; ++i) __builtin_amdgcn_s_sleep(16);
+    }
+}
+
 // Epilogue twiddle for the next pass: element at position P (within the
 // transform axis, P = global_pos >> shift) gets W_{s*R}^{c*rev(j)} with
 // c = P mod s, j = (P / s) mod R and rev the two-digit DIT reversal of the

@@ -19,6 +19,7 @@
 #include "fft_blocks.hpp"
 #include "fft_kernels.hpp"
 
+#include <map>
 #include <mutex>
 
 namespace tcfft {
@@ -149,11 +150,13 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
 
 template <int T, bool EPI, int SIGN>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows(u32* __restrict__ buf, u32 n_tiles,
-                                              const float2* __restrict__ w4096, EpiParams epi) {
+                                              const float2* __restrict__ w4096, EpiParams epi,
+                                              StartSched sched) {
     const int lane = threadIdx.x & 63;
     const u32 wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const u32 tile0 = wave * 16u;
     if (tile0 >= n_tiles) return;
+    start_delay(sched);
     u32* const base = buf + (size_t)tile0 * 256 + 16 * (lane & 15) + 4 * (lane >> 4);
     rows_body<T, EPI, SIGN>([&](int t) { return base + t * 256; }, min(16u, n_tiles - tile0), tile0, w4096, epi);
 }
@@ -436,42 +439,76 @@ __global__ __launch_bounds__(256) void k_cols_small(u32* __restrict__ buf, u32 s
 template <int W, bool EPI, int SIGN>
 __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big(u32* __restrict__ buf, const float2* __restrict__ w4096,
                                                 const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
-                                                u32 wn_lo_bits, EpiParams epi, BigSched sched) {
+                                                u32 wn_lo_bits, EpiParams epi, StartSched sched) {
     extern __shared__ u32 lds[];
-    if (sched.delay && blockIdx.x < sched.resident) {
-        const u32 quanta = ((blockIdx.x >> 3) % sched.groups) * sched.delay;
-        for (u32 i = 0; i < quanta; ++i) __builtin_amdgcn_s_sleep(16);
-    }
+    start_delay(sched);
     big_body<W, BigCfg<W>::ROUNDS, EPI, SIGN>(buf + (size_t)blockIdx.x * BigCfg<W>::N, threadIdx.x >> 6, lds,
                                              blockIdx.x, w4096, wn_hi, wn_lo, wn_lo_bits, epi);
 }
 
 // ----------------------------------------------------------------------------
 // Host-side launchers (called by the pass driver).
+// One-time set-up of a kernel, cached by the kernel's address (every
+// template instance of a kernel has the same pointer type, so a static per
+// type would be shared by all of them): raise its dynamic-LDS limit to
+// `lds` bytes and count the workgroups of `threads` threads resident at once.
+struct KernelInfo {
+    hipError_t lds_status;
+    u32 resident;
+};
+
+template <class K>
+static KernelInfo kernel_info(K kernel, int threads, size_t lds) {
+    static std::mutex mu;
+    static std::map<const void*, KernelInfo> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto it = cache.find((const void*)kernel);
+    if (it != cache.end()) return it->second;
+    KernelInfo info{hipSuccess, 0};
+    if (lds > 0)
+        info.lds_status = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kernel, threads, lds) == hipSuccess)
+        info.resident = (u32)(cus * per_cu);
+    cache.emplace((const void*)kernel, info);
+    return info;
+}
+
 template <int T, int SIGN>
-static hipError_t launch_rows_t(u32* buf, u32 n_tiles, const float2* w4096, const EpiParams* epi, hipStream_t st) {
+static hipError_t launch_rows_t(u32* buf, u32 n_tiles, const float2* w4096, const EpiParams* epi, StartSched sched,
+                                hipStream_t st) {
     const u32 waves = (n_tiles + 15) / 16;
     const u32 blocks = (waves + 3) / 4;
-    if (epi) hipLaunchKernelGGL((k_rows<T, true, SIGN>), dim3(blocks), dim3(256), 0, st, buf, n_tiles, w4096, *epi);
-    else hipLaunchKernelGGL((k_rows<T, false, SIGN>), dim3(blocks), dim3(256), 0, st, buf, n_tiles, w4096, EpiParams{});
+    if (sched.groups == 0) sched.delay = 0;
+    if (epi) {
+        auto k = k_rows<T, true, SIGN>;
+        if (sched.delay) sched.resident = kernel_info(k, 256, 0).resident;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, st, buf, n_tiles, w4096, *epi, sched);
+    } else {
+        auto k = k_rows<T, false, SIGN>;
+        if (sched.delay) sched.resident = kernel_info(k, 256, 0).resident;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, st, buf, n_tiles, w4096, EpiParams{}, sched);
+    }
     return hipGetLastError();
 }
 
 template <int T>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
-                                hipStream_t st) {
-    return sign < 0 ? launch_rows_t<T, -1>(buf, n_tiles, w4096, epi, st)
-                    : launch_rows_t<T, +1>(buf, n_tiles, w4096, epi, st);
+                                StartSched sched, hipStream_t st) {
+    return sign < 0 ? launch_rows_t<T, -1>(buf, n_tiles, w4096, epi, sched, st)
+                    : launch_rows_t<T, +1>(buf, n_tiles, w4096, epi, sched, st);
 }
 
 hipError_t launch_rows(int T, u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
-                       hipStream_t st) {
+                       StartSched sched, hipStream_t st) {
     switch (T) {
-    case 1: return launch_rows_t<1>(buf, n_tiles, sign, w4096, epi, st);
-    case 2: return launch_rows_t<2>(buf, n_tiles, sign, w4096, epi, st);
-    case 4: return launch_rows_t<4>(buf, n_tiles, sign, w4096, epi, st);
-    case 8: return launch_rows_t<8>(buf, n_tiles, sign, w4096, epi, st);
-    case 16: return launch_rows_t<16>(buf, n_tiles, sign, w4096, epi, st);
+    case 1: return launch_rows_t<1>(buf, n_tiles, sign, w4096, epi, sched, st);
+    case 2: return launch_rows_t<2>(buf, n_tiles, sign, w4096, epi, sched, st);
+    case 4: return launch_rows_t<4>(buf, n_tiles, sign, w4096, epi, sched, st);
+    case 8: return launch_rows_t<8>(buf, n_tiles, sign, w4096, epi, sched, st);
+    case 16: return launch_rows_t<16>(buf, n_tiles, sign, w4096, epi, sched, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -518,35 +555,9 @@ hipError_t launch_cols(int R, u32* buf, u32 s_log2, u32 outer, int sign, const f
     }
 }
 
-// Raise the dynamic-LDS limit of a kernel once per process (thread-safe).
-template <class K>
-static hipError_t set_lds_once(K kernel, size_t bytes) {
-    static std::once_flag flag;
-    static hipError_t status = hipSuccess;
-    std::call_once(flag, [&] {
-        status = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
-    });
-    return status;
-}
-
-// Workgroups of kernel k resident at once on the device (cached per kernel).
-template <class K>
-static u32 resident_blocks(K kernel, int threads, size_t lds) {
-    static std::once_flag flag;
-    static u32 blocks = 0;
-    std::call_once(flag, [&] {
-        int dev = 0, cus = 0, per_cu = 0;
-        if (hipGetDevice(&dev) == hipSuccess &&
-            hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-            hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kernel, threads, lds) == hipSuccess)
-            blocks = (u32)(cus * per_cu);
-    });
-    return blocks;
-}
-
 template <int W, int SIGN>
 static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, const float2* wn_hi,
-                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, BigSched sched,
+                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, StartSched sched,
                                hipStream_t st) {
     const size_t lds = BigCfg<W>::LDS_WORDS * sizeof(u32);
     if (sched.groups == 0) sched.delay = 0;
@@ -556,9 +567,9 @@ static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, con
         if constexpr (W == 16) {
             if (epi->shift != 0 || epi->s_log2 != 16) return hipErrorInvalidValue;
             auto k = k_big<W, true, SIGN>;
-            hipError_t e = set_lds_once(k, lds);
-            if (e != hipSuccess) return e;
-            if (sched.delay) sched.resident = resident_blocks(k, 64 * W, lds);
+            const KernelInfo ki = kernel_info(k, 64 * W, lds);
+            if (ki.lds_status != hipSuccess) return ki.lds_status;
+            if (sched.delay) sched.resident = ki.resident;
             hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits, *epi,
                                sched);
         } else {
@@ -566,9 +577,9 @@ static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, con
         }
     } else {
         auto k = k_big<W, false, SIGN>;
-        hipError_t e = set_lds_once(k, lds);
-        if (e != hipSuccess) return e;
-        if (sched.delay) sched.resident = resident_blocks(k, 64 * W, lds);
+        const KernelInfo ki = kernel_info(k, 64 * W, lds);
+        if (ki.lds_status != hipSuccess) return ki.lds_status;
+        if (sched.delay) sched.resident = ki.resident;
         hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, buf, w4096, wn_hi, wn_lo, lo_bits,
                            EpiParams{}, sched);
     }
@@ -577,14 +588,14 @@ static hipError_t launch_big_t(u32* buf, u32 n_signals, const float2* w4096, con
 
 template <int W>
 static hipError_t launch_big_t(u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
-                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, BigSched sched,
+                               const float2* wn_lo, u32 lo_bits, const EpiParams* epi, StartSched sched,
                                hipStream_t st) {
     return sign < 0 ? launch_big_t<W, -1>(buf, n_signals, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st)
                     : launch_big_t<W, +1>(buf, n_signals, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st);
 }
 
 hipError_t launch_big(int W, u32* buf, u32 n_signals, int sign, const float2* w4096, const float2* wn_hi,
-                      const float2* wn_lo, u32 lo_bits, const EpiParams* epi, BigSched sched, hipStream_t st) {
+                      const float2* wn_lo, u32 lo_bits, const EpiParams* epi, StartSched sched, hipStream_t st) {
     switch (W) {
     case 2: return launch_big_t<2>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st);
     case 4: return launch_big_t<4>(buf, n_signals, sign, w4096, wn_hi, wn_lo, lo_bits, epi, sched, st);
@@ -672,8 +683,8 @@ static hipError_t launch_multi_t(u32* buf, const MultiTask* tasks, u32 n_tasks, 
                                  const float2* w4096, const MultiTables& tb, hipStream_t st) {
     constexpr size_t lds = 16 * 2048 * sizeof(u32);  // 128 KiB
     auto k = k_multi<SIGN>;
-    hipError_t e = set_lds_once(k, lds);
-    if (e != hipSuccess) return e;
+    const KernelInfo ki = kernel_info(k, 1024, lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
     hipLaunchKernelGGL(k, dim3(grid), dim3(1024), lds, st, buf, tasks, n_tasks, offsets, w4096, tb);
     return hipGetLastError();
 }

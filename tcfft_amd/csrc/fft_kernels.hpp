@@ -19,12 +19,13 @@ struct EpiParams {
     uint32_t ra_log2;  // first radix inside the next pass (DIT order)
 };
 
-// Start schedule of k_big.  Workgroups of the first resident wave (b <
-// resident) are put in `groups` groups by their index within an XCD
-// (b >> 3, blocks go round-robin over the 8 XCDs); group k sleeps k * delay
-// quanta of 1024 cycles before its first load, so the groups' load, compute
-// and store phases overlap instead of running in lockstep.  delay = 0: none.
-struct BigSched {
+// Start schedule of the first-pass kernels (k_rows, k_big).  Workgroups of
+// the first resident wave (b < resident) are put in `groups` groups by their
+// index within an XCD (b >> 3, blocks go round-robin over the 8 XCDs); group
+// k sleeps k * delay quanta of 1024 cycles before its first load, so the
+// groups' load, compute and store phases overlap instead of running in
+// lockstep.  delay = 0: none.
+struct StartSched {
     uint32_t delay;
     uint32_t groups;
     uint32_t resident;
@@ -32,12 +33,12 @@ struct BigSched {
 
 // First pass: contiguous lines of 256*T elements (T = 1, 2, 4, 8, 16).
 hipError_t launch_rows(int T, uint32_t* buf, uint32_t n_tiles, int sign, const float2* w4096,
-                       const EpiParams* epi, hipStream_t st);
+                       const EpiParams* epi, StartSched sched, hipStream_t st);
 
 // Single-pass transform of N = 4096*W (W = 2, 4, 8, 16) per workgroup of W
 // waves.  wn_hi/wn_lo: two-level table of W_N (lo_bits split).
 hipError_t launch_big(int W, uint32_t* buf, uint32_t n_signals, int sign, const float2* w4096, const float2* wn_hi,
-                      const float2* wn_lo, uint32_t lo_bits, const EpiParams* epi, BigSched sched, hipStream_t st);
+                      const float2* wn_lo, uint32_t lo_bits, const EpiParams* epi, StartSched sched, hipStream_t st);
 
 // Split-precision N = 65536 transform of FP32 complex data (in != out), as
 // two passes per chunk of `chunk` signals (0 = all at once).

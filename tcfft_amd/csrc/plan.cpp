@@ -184,7 +184,8 @@ struct tcfft_plan_s {
     size_t workspace_bytes = 0;
     size_t total = 0;  // elements in the buffer
     int split_chunk = 128;  // signals per pass pair in tcfft_exec_split
-    int big_delay = -1;   // start schedule of the single-pass kernel: -1 = per-width default
+    int cus = 0;          // compute units of the plan's device
+    int big_delay = -1;   // start schedule of the first-pass kernels: -1 = default
     int big_groups = 3;
 };
 
@@ -245,6 +246,8 @@ int table_pair(tcfft_plan_s* p, uint32_t M, std::pair<float2*, float2*>& out) {
 int finish_plan(tcfft_plan_s* p) {
     hipError_t e = hipGetDevice(&p->device);
     if (e != hipSuccess) return hip_fail(e);
+    e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device);
+    if (e != hipSuccess) return hip_fail(e);
     int rc = upload(p, host_table(12, 1, 4096), &p->w4096);
     if (rc) return rc;
     for (Pass& ps : p->lay.passes) {
@@ -284,9 +287,19 @@ bool valid(tcfft_plan p) { return p && p->magic == kMagic; }
 // Start schedule of the single-pass kernel of width W.  Defaults measured on
 // MI355X (3 groups; 2^15 x 512: 26.2 -> 24.0 us with delay 1, 2^16 x 256:
 // 28.9 -> 26.5 us with delay 2, 2^14 x 2048: 53.9 -> 50.4 us with delay 1).
-tcfft::BigSched big_sched(const tcfft_plan_s* p, int W) {
+tcfft::StartSched big_sched(const tcfft_plan_s* p, int W) {
     const int delay = p->big_delay >= 0 ? p->big_delay : (W == 4 || W == 8 ? 1 : 2);
-    return tcfft::BigSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
+    return tcfft::StartSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
+}
+
+// Start schedule of the contiguous-row first pass.  By default a delay of 1
+// only when the grid is at most one 4-wave workgroup per CU (4096 x 1024:
+// 10.0 -> 9.2 us); with more workgroups per CU they desynchronise by
+// themselves and a delay only costs time (1024 x 8192: 15.9 -> 16.2 us).
+tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps) {
+    const uint32_t blocks = ((ps.n_tiles + 15u) / 16u + 3u) / 4u;
+    const int delay = p->big_delay >= 0 ? p->big_delay : (blocks <= (uint32_t)p->cus ? 1 : 0);
+    return tcfft::StartSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
 }
 
 int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t first = 0, size_t count = ~(size_t)0) {
@@ -295,7 +308,7 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
         const Pass& ps = p->lay.passes[i];
         const EpiParams* epi = ps.has_epi ? &ps.epi : nullptr;
         hipError_t e;
-        if (ps.kind == 0) e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, st);
+        if (ps.kind == 0) e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, rows_sched(p, ps), st);
         else if (ps.kind == 2)
             e = tcfft::launch_big(ps.radix, buf, ps.n_tiles, sign, p->w4096, ps.big_hi, ps.big_lo, ps.big_M_log2 / 2,
                                   epi, big_sched(p, ps.radix), st);
@@ -538,8 +551,6 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
                         float* best_ms) {
     if (!valid(plan)) return TCFFT_INVALID_PLAN;
     if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
-    bool has_big = false;
-    for (const Pass& ps : plan->lay.passes) has_big |= ps.kind == 2;
     DeviceGuard guard(plan->device);
     hipStream_t st = (hipStream_t)stream;
     hipEvent_t a, b;
@@ -553,15 +564,10 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
         float ms;
     };
     std::vector<Cand> cands;
-    if (has_big) {
-        const int g3[] = {0, 1, 2, 3};
-        for (int d : g3) cands.push_back({3, d, 1e30f});
-        cands.push_back({5, 1, 1e30f});
-        cands.push_back({5, 2, 1e30f});
-        cands.push_back({7, 1, 1e30f});
-    } else {
-        cands.push_back({plan->big_groups, plan->big_delay, 1e30f});
-    }
+    for (int d = 0; d <= 3; ++d) cands.push_back({3, d, 1e30f});
+    cands.push_back({5, 1, 1e30f});
+    cands.push_back({5, 2, 1e30f});
+    cands.push_back({7, 1, 1e30f});
     const int saved_groups = plan->big_groups, saved_delay = plan->big_delay;
     const int reps = 10;
     int rc = TCFFT_SUCCESS;

@@ -1,4 +1,4 @@
-"""Start schedule of the single-pass kernel (plan options TCFFT_OPT_BIG_DELAY /
+"""Start schedule of the first-pass kernels (plan options TCFFT_OPT_BIG_DELAY /
 TCFFT_OPT_BIG_GROUPS and tcfft_plan_autotune): every schedule gives output
 bit-identical to the unstaggered launch, the default and tuned schedules match
 FP64 numpy (rel-L2 <= 1e-3 up to 2^16, 1.5e-3 above), and bad option values
@@ -35,9 +35,11 @@ def rel_l2(t, want):
     return float(np.linalg.norm(got - want) / np.linalg.norm(want))
 
 
-# sizes whose first pass is the single-pass kernel of each width, and a
-# longer transform whose first pass folds the next pass's twiddle
-@pytest.mark.parametrize("n,batch", [(8192, 300), (16384, 260), (32768, 270), (65536, 259), (1 << 20, 3)])
+# sizes whose first pass is the single-pass kernel of each width, a longer
+# transform whose first pass folds the next pass's twiddle, and contiguous-row
+# first passes (one workgroup per CU and more)
+@pytest.mark.parametrize("n,batch", [(8192, 300), (16384, 260), (32768, 270), (65536, 259), (1 << 20, 3),
+                                     (4096, 1024), (1024, 300), (256, 4099)])
 def test_schedules_bit_identical(n, batch):
     note(f"[gpu] schedule n={n} batch={batch}")
     x, d = dropin_input(n, batch, n + batch)
@@ -71,8 +73,7 @@ def test_autotune_then_parity(n, batch):
     groups, delay, ms = plan.autotune(scratch.data_ptr())
     torch.cuda.synchronize()
     assert ms > 0
-    if n == 65536:
-        assert (groups, delay) in {(3, 0), (3, 1), (3, 2), (3, 3), (5, 1), (5, 2), (7, 1)}
+    assert (groups, delay) in {(3, 0), (3, 1), (3, 2), (3, 3), (5, 1), (5, 2), (7, 1)}
     t = d.clone()
     plan.exec(t.data_ptr())
     torch.cuda.synchronize()
