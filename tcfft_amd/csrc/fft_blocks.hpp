@@ -22,13 +22,22 @@ __device__ __forceinline__ void start_delay(const StartSched& s) {
 // transform axis, P = global_pos >> shift) gets W_{s*R}^{c*rev(j)} with
 // c = P mod s, j = (P / s) mod R and rev the two-digit DIT reversal of the
 // next pass (radix ra first, then R/ra).  W_M^m = hi[m >> lo_bits]*lo[m & lo_mask].
+// Digit reversal of a row index j (0 <= j < R) of the next pass, whose DIT
+// digits are [2^ra, 2^rb, 2^rc] (rc = r - ra - rb, often 0):
+// j = j0 + 2^ra*j1 + 2^(ra+rb)*j2 -> j2 + 2^rc*j1 + 2^(rc+rb)*j0.
+__device__ __forceinline__ u32 epi_rev_row(const EpiParams& e, u32 j) {
+    const u32 rc = e.r_log2 - e.ra_log2 - e.rb_log2;
+    const u32 j0 = j & ((1u << e.ra_log2) - 1u);
+    const u32 j1 = (j >> e.ra_log2) & ((1u << e.rb_log2) - 1u);
+    const u32 j2 = j >> (e.ra_log2 + e.rb_log2);
+    return j2 | (j1 << rc) | (j0 << (rc + e.rb_log2));
+}
+
 __device__ __forceinline__ cf epi_twiddle(const EpiParams& e, u32 gpos, float sgn) {
     const u32 P = gpos >> e.shift;
     const u32 c = P & ((1u << e.s_log2) - 1u);
     const u32 j = (P >> e.s_log2) & ((1u << e.r_log2) - 1u);
-    const u32 j0 = j & ((1u << e.ra_log2) - 1u);
-    const u32 j1 = j >> e.ra_log2;
-    const u32 rj = j1 | (j0 << (e.r_log2 - e.ra_log2));
+    const u32 rj = epi_rev_row(e, j);
     const u32 m = (c * rj) & e.m_mask;
     const cf a = twiddle(e.hi, m >> e.lo_bits, sgn);
     const cf b = twiddle(e.lo, m & ((1u << e.lo_bits) - 1u), sgn);
@@ -42,9 +51,7 @@ __device__ __forceinline__ cf epi_twiddle(const EpiParams& e, u32 gpos, float sg
 // per-row factor, both powers of one root.  epi_rev gives rev(blk mod R'),
 // epi_w gives W_M^m from the two-level table.
 __device__ __forceinline__ u32 epi_rev(const EpiParams& e, u32 blk) {
-    const u32 j = blk & ((1u << e.r_log2) - 1u);
-    const u32 j0 = j & ((1u << e.ra_log2) - 1u);
-    return (j >> e.ra_log2) | (j0 << (e.r_log2 - e.ra_log2));
+    return epi_rev_row(e, blk & ((1u << e.r_log2) - 1u));
 }
 
 __device__ __forceinline__ cf epi_w(const EpiParams& e, u32 m, float sgn) {
@@ -184,9 +191,7 @@ __device__ __forceinline__ void big_tw_load(BigTw& bt, int lane, int w, u32 sig,
     const int c16 = lane & 15, g = lane >> 4;
     u32 e_w = (u32)w, rj = 0u;
     if constexpr (EPI) {
-        const u32 j = sig & ((1u << epi.r_log2) - 1u);
-        const u32 j0 = j & ((1u << epi.ra_log2) - 1u);
-        rj = (j >> epi.ra_log2) | (j0 << (epi.r_log2 - epi.ra_log2));
+        rj = epi_rev(epi, sig);
         e_w = ((u32)w << epi.r_log2) + rj;
     }
     const float2* q_hi = EPI ? epi.hi : wn_hi;

@@ -390,6 +390,124 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_cols_q(u32* __restric
 }
 
 // ----------------------------------------------------------------------------
+// k_cols_big<W>: the strided pass of radix R = 256*W (W = 2, 4, 8) at element
+// step s in one HBM round trip, as the last pass of its axis (no epilogue).
+// A workgroup of W waves owns 16 adjacent columns of one [R][s] block.  Rows
+// j = j0 + 16*j1 + 256*j2 are digit-reversed for [16, 16, W]; wave w loads
+// the 256 rows with j2 = w (lane (c16, g) word i of tile j1: row j0 = 4g+i,
+// column c16) and computes their 256-point DFT times the radix-W stage's
+// twiddle W_{256W}^{m*w}, m = k0 + 16*k1, at no per-element cost:
+//   stage 0: DFT-16 over j0 (B-form) -> (g,i) = k0, times W_256^{k0*j1} *
+//            W_{256W}^{k0*w} (a recurrence over the tiles j1 that starts at
+//            W_{256W}^{k0*w}); transpose to tile = k0, (g,i) = j1;
+//   stage 1: DFT-16 over j1 whose operand rows are scaled by
+//            W_{256W}^{16*k1*w} -> tile k0, (g,i) = k1.
+// Exchange through LDS laid out [source wave][tile][lane][4 words] per round
+// (b128 both ways, conflict-free), each wave taking TPW tiles of every source
+// per round, then the radix-W stage over the sources (VALU) writes row
+// k0 + 16*k1 + 256*k2.  Workgroups are mapped so that those on one XCD
+// (block mod 8) take adjacent column groups, whose 64-byte row segments share
+// 128-byte lines in that XCD's L2.
+template <int W>
+struct ColsBigCfg {
+    static constexpr int ROUNDS = W >= 4 ? 2 : 1;
+    static constexpr int TPR = 16 / ROUNDS;  // tiles exchanged per round
+    static constexpr int TPW = TPR / W;      // tiles per reader wave per round
+    static constexpr int LDS_WORDS = W * TPR * 256;
+};
+
+template <int W, int SIGN>
+__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __restrict__ buf, u32 s_log2,
+                                                                     u32 n_groups,
+                                                                     const float2* __restrict__ w4096) {
+    using C = ColsBigCfg<W>;
+    static_assert(W == 2 || W == 4 || W == 8, "k_cols_big: W = 2, 4, 8");
+    extern __shared__ u32 lds[];
+    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+    const u32 w = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    u32 grp = blockIdx.x;
+    if ((n_groups & 7u) == 0u) grp = (blockIdx.x & 7u) * (n_groups >> 3) + (blockIdx.x >> 3);
+    const u32 cg_log2 = s_log2 - 4u;  // column groups of 16 per block: s / 16
+    const u32 blk = grp >> cg_log2;
+    const u32 col0 = (grp & ((1u << cg_log2) - 1u)) * 16u;
+    const size_t blk_base = (((size_t)blk * (256u * W)) << s_log2) + col0 + (u32)c16;
+
+    // table values before the data (global loads complete in issue order)
+    float2 t_step[4], t_rec0[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const u32 k0 = 4u * g + i;
+        t_step[i] = w4096[16u * k0];                    // W_256^{k0}
+        t_rec0[i] = w4096[(k0 * w * (16u / W)) & 4095u];  // W_{256W}^{k0*w}
+    }
+    const float2 t_s1 = w4096[((u32)c16 * w * (256u / W)) & 4095u];  // W_{256W}^{16*c16*w}
+    __builtin_amdgcn_sched_barrier(0);
+    u32 d[16][4];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[t][i] = buf[blk_base + ((size_t)(16u * t + 4u * g + i + 256u * w) << s_log2)];
+    __builtin_amdgcn_sched_barrier(0);
+
+    const Dft16Op op = make_dft16(lane, SIGN);
+    {
+        cf step[4], rec[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            step[i] = cf{t_step[i].x, sg * t_step[i].y};
+            rec[i] = cf{t_rec0[i].x, sg * t_rec0[i].y};
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            f32x4 re, im;
+            dft16_b(d[t], op, re, im);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                d[t][i] = pack(cmul(cf{re[i], im[i]}, rec[i]));
+                if (t < 15) rec[i] = cmul(rec[i], step[i]);
+            }
+        }
+    }
+    u32 e[16][4];
+    transpose_gi_tile(d, e);  // tile = k0, (g,i) = j1
+    const Dft16Op op1 = make_dft16_scaled(lane_id(), SIGN, cf{t_s1.x, sg * t_s1.y});
+#pragma unroll
+    for (int r = 0; r < C::ROUNDS; ++r) {
+        if (r) __syncthreads();  // the previous round's reads are done
+#pragma unroll
+        for (int tt = 0; tt < C::TPR; ++tt) {
+            f32x4 re, im;
+            dft16_b(e[r * C::TPR + tt], op1, re, im);
+            u32 q[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) q[i] = pack(cf{re[i], im[i]});
+            store4(lds + ((w * C::TPR + tt) * 64u + (u32)lane) * 4u, q);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int ti = 0; ti < C::TPW; ++ti) {
+            const u32 tt = w * C::TPW + ti;
+            u32 v[W][4];
+#pragma unroll
+            for (int src = 0; src < W; ++src) load4(v[src], lds + (((u32)src * C::TPR + tt) * 64u + (u32)lane) * 4u);
+            const u32 k0 = (u32)r * C::TPR + tt;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                cf x[W];
+#pragma unroll
+                for (int src = 0; src < W; ++src) x[src] = unpack(v[src][i]);
+                small_dft<W>(x, SIGN);
+                const u32 row = k0 + 16u * (4u * g + i);
+#pragma unroll
+                for (int k2 = 0; k2 < W; ++k2) buf[blk_base + ((size_t)(row + 256u * k2) << s_log2)] = pack(x[k2]);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // k_cols_small<R>: strided radix-R pass (R = 2, 4, 8), FP32 VALU, four adjacent
 // columns per lane (16-byte loads).  Rows are in natural order within the pass.
 template <int R, bool EPI>
@@ -528,6 +646,30 @@ static hipError_t launch_cols_t(u32* buf, u32 s_log2, u32 outer, int sign, const
     if (epi) hipLaunchKernelGGL((k_cols<T, true>), dim3(blocks), dim3(256), 0, st, buf, s_log2, waves, sign, w4096, *epi);
     else hipLaunchKernelGGL((k_cols<T, false>), dim3(blocks), dim3(256), 0, st, buf, s_log2, waves, sign, w4096, EpiParams{});
     return hipGetLastError();
+}
+
+template <int W, int SIGN>
+static hipError_t launch_cols_big_t(u32* buf, u32 s_log2, u32 outer, const float2* w4096, hipStream_t st) {
+    const size_t lds = ColsBigCfg<W>::LDS_WORDS * sizeof(u32);
+    auto k = k_cols_big<W, SIGN>;
+    const KernelInfo ki = kernel_info(k, 64 * W, lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    const u32 n_groups = outer << (s_log2 - 4u);
+    hipLaunchKernelGGL(k, dim3(n_groups), dim3(64 * W), lds, st, buf, s_log2, n_groups, w4096);
+    return hipGetLastError();
+}
+
+hipError_t launch_cols_big(int W, u32* buf, u32 s_log2, u32 outer, int sign, const float2* w4096, hipStream_t st) {
+    if (s_log2 < 4u || ((size_t)outer << (s_log2 - 4u)) > 0x7fffffffu) return hipErrorInvalidValue;
+    switch (W * 2 + (sign < 0 ? 0 : 1)) {
+    case 4: return launch_cols_big_t<2, -1>(buf, s_log2, outer, w4096, st);
+    case 5: return launch_cols_big_t<2, +1>(buf, s_log2, outer, w4096, st);
+    case 8: return launch_cols_big_t<4, -1>(buf, s_log2, outer, w4096, st);
+    case 9: return launch_cols_big_t<4, +1>(buf, s_log2, outer, w4096, st);
+    case 16: return launch_cols_big_t<8, -1>(buf, s_log2, outer, w4096, st);
+    case 17: return launch_cols_big_t<8, +1>(buf, s_log2, outer, w4096, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int R>

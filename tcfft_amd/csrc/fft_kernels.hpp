@@ -17,6 +17,7 @@ struct EpiParams {
     uint32_t s_log2;   // next pass step, in axis positions
     uint32_t r_log2;   // next pass radix
     uint32_t ra_log2;  // first radix inside the next pass (DIT order)
+    uint32_t rb_log2;  // second radix inside the next pass (the rest is the third)
 };
 
 // Start schedule of the first-pass kernels (k_rows, k_big).  Workgroups of
@@ -39,6 +40,11 @@ hipError_t launch_rows(int T, uint32_t* buf, uint32_t n_tiles, int sign, const f
 // waves.  wn_hi/wn_lo: two-level table of W_N (lo_bits split).
 hipError_t launch_big(int W, uint32_t* buf, uint32_t n_signals, int sign, const float2* w4096, const float2* wn_hi,
                       const float2* wn_lo, uint32_t lo_bits, const EpiParams* epi, StartSched sched, hipStream_t st);
+
+// Last strided pass of its axis with radix R = 256*W (W = 2, 4, 8) at element
+// step 2^s_log2 over `outer` [R][s] blocks, in one HBM round trip.
+hipError_t launch_cols_big(int W, uint32_t* buf, uint32_t s_log2, uint32_t outer, int sign, const float2* w4096,
+                           hipStream_t st);
 
 // Split-precision N = 65536 transform of FP32 complex data (in != out), as
 // two passes per chunk of `chunk` signals (0 = all at once).

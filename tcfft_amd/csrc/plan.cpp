@@ -29,7 +29,8 @@ constexpr uint32_t kMagic = 0x7cff7a11u;
 thread_local int g_last_hip_error = 0;
 
 struct Pass {
-    int kind = 0;          // 0: contiguous rows, 1: strided columns, 2: single-pass big lines
+    int kind = 0;          // 0: contiguous rows, 1: strided columns, 2: single-pass big lines,
+                           // 3: strided columns of radix 256*W in one round trip
     int radix = 0;         // rows: T (line = 256*T); columns: R; big: W (line = 4096*W)
     uint32_t n_tiles = 0;  // rows: 256-element tiles; big: number of lines
     uint32_t s_log2 = 0;   // columns: element step
@@ -56,8 +57,11 @@ int ilog2(uint32_t v) {
 }
 bool is_pow2(long v) { return v > 0 && (v & (v - 1)) == 0; }
 
-// Split `bits` into pass radices of at most 2^8.
+// Split `bits` into strided pass radices: one pass of radix 2^9..2^11
+// (k_cols_big, 256*W with W = 2, 4, 8) when 9 to 11 bits remain, else
+// passes of at most 2^8.
 std::vector<int> split_bits(int bits) {
+    if (bits >= 9 && bits <= 11) return {1 << bits};
     std::vector<int> out;
     while (bits > 0) {
         int b = bits > 8 ? 8 : bits;
@@ -67,14 +71,30 @@ std::vector<int> split_bits(int bits) {
     return out;
 }
 
-// DIT radix digits consumed inside one strided pass of radix R.
+// DIT radix digits consumed inside one strided pass of radix R:
+// [R] (R < 16), [16, R/16] (R <= 256), [16, 16, R/256] (R > 256).
 void pass_digits(int R, std::vector<int>& rad) {
     if (R >= 16) {
         rad.push_back(16);
-        if (R > 16) rad.push_back(R / 16);
+        if (R > 256) {
+            rad.push_back(16);
+            rad.push_back(R / 256);
+        } else if (R > 16) {
+            rad.push_back(R / 16);
+        }
     } else {
         rad.push_back(R);
     }
+}
+
+// A strided pass of radix R: kind 3 (one round trip) above 256.
+Pass strided_pass(int R, uint32_t s_log2, uint32_t outer) {
+    Pass p;
+    p.kind = R > 256 ? 3 : 1;
+    p.radix = R;
+    p.s_log2 = s_log2;
+    p.outer = outer;
+    return p;
 }
 
 // Epilogue description for a next pass (radix R at axis step s, shift).
@@ -84,6 +104,7 @@ void set_epi(Pass& p, uint32_t s_log2_axis, int R_next, uint32_t shift) {
     p.epi.s_log2 = s_log2_axis;
     p.epi.r_log2 = (uint32_t)ilog2((uint32_t)R_next);
     p.epi.ra_log2 = R_next >= 16 ? 4u : p.epi.r_log2;
+    p.epi.rb_log2 = R_next > 256 ? 4u : p.epi.r_log2 - p.epi.ra_log2;
     p.epi_M_log2 = s_log2_axis + p.epi.r_log2;
     p.epi.m_mask = (p.epi_M_log2 >= 32) ? 0xffffffffu : ((1u << p.epi_M_log2) - 1u);
     p.epi.lo_bits = p.epi_M_log2 / 2;
@@ -118,13 +139,8 @@ int build_axis_rows(int n, long lines, std::vector<int>& rad, std::vector<Pass>&
     uint32_t s_log2 = (uint32_t)ilog2((uint32_t)L);
     for (int R : split_bits(nb - (int)s_log2)) {
         set_epi(passes.back(), s_log2, R, 0);
-        Pass p;
-        p.kind = 1;
-        p.radix = R;
-        p.s_log2 = s_log2;
-        p.outer = (uint32_t)((long)n * lines / ((long)R << s_log2));
         pass_digits(R, rad);
-        passes.push_back(p);
+        passes.push_back(strided_pass(R, s_log2, (uint32_t)((long)n * lines / ((long)R << s_log2))));
         s_log2 += (uint32_t)ilog2((uint32_t)R);
     }
     return 0;
@@ -159,13 +175,9 @@ int build_layout_2d(int nx, int ny, int batch, Layout& out) {
     for (int R : split_bits(ilog2((uint32_t)nx))) {
         if (!first) set_epi(out.passes.back(), sx_log2, R, ny_log2);
         first = false;
-        Pass p;
-        p.kind = 1;
-        p.radix = R;
-        p.s_log2 = ny_log2 + sx_log2;
-        p.outer = (uint32_t)((long)nx * batch / ((long)R << sx_log2));
         pass_digits(R, out.rad_x);
-        out.passes.push_back(p);
+        out.passes.push_back(
+            strided_pass(R, ny_log2 + sx_log2, (uint32_t)((long)nx * batch / ((long)R << sx_log2))));
         sx_log2 += (uint32_t)ilog2((uint32_t)R);
     }
     return TCFFT_SUCCESS;
@@ -312,6 +324,9 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
         else if (ps.kind == 2)
             e = tcfft::launch_big(ps.radix, buf, ps.n_tiles, sign, p->w4096, ps.big_hi, ps.big_lo, ps.big_M_log2 / 2,
                                   epi, big_sched(p, ps.radix), st);
+        else if (ps.kind == 3)
+            e = epi ? hipErrorInvalidValue
+                    : tcfft::launch_cols_big(ps.radix / 256, buf, ps.s_log2, ps.outer, sign, p->w4096, st);
         else e = tcfft::launch_cols(ps.radix, buf, ps.s_log2, ps.outer, sign, p->w4096, epi, st);
         if (e != hipSuccess) return hip_fail(e);
     }

@@ -107,11 +107,13 @@ def test_multipass_forward_vs_oracle(n, batch):
 @pytest.mark.parametrize("n,batch,direction", [(1 << 19, 1, tcfft_amd.FORWARD), (1 << 20, 2, tcfft_amd.FORWARD),
                                                (1 << 20, 1, tcfft_amd.INVERSE), (1 << 21, 1, tcfft_amd.FORWARD),
                                                (1 << 22, 1, tcfft_amd.FORWARD),
-                                               (1 << 24, 1, tcfft_amd.FORWARD), (1 << 25, 1, tcfft_amd.INVERSE)])
+                                               (1 << 24, 1, tcfft_amd.FORWARD), (1 << 25, 1, tcfft_amd.INVERSE),
+                                               (1 << 26, 1, tcfft_amd.FORWARD), (1 << 27, 1, tcfft_amd.INVERSE)])
 def test_large_vs_numpy(n, batch, direction):
     """Sizes above 2^18 (every radix of the pass after the 65536-point
-    first pass) against numpy's FP64 FFT; the oracle is pinned to numpy at
-    smaller sizes by the CPU tests."""
+    first pass, including the single-pass radices 512..2048 whose twiddle the
+    first pass folds over three digits) against numpy's FP64 FFT; the oracle
+    is pinned to numpy at smaller sizes by the CPU tests."""
     x = fp16_input((batch, n), seed=n + 11)
     got = run_1d(x, direction)
     ref = np.fft.fft(x, axis=-1) if direction == tcfft_amd.FORWARD else np.fft.ifft(x, axis=-1) * n
@@ -150,8 +152,10 @@ def test_golden_vectors(golden_dir, n):
         assert oracle.rel_l2(run_1d(x, tcfft_amd.INVERSE), d["adjoint"]) <= TOL
 
 
+# nx = 512, 1024, 2048: the column axis in one pass of radix 256*W
 @pytest.mark.parametrize("shape,batch", [((256, 256), 2), ((512, 1024), 1), ((1024, 1024), 1), ((16, 256), 3),
-                                         ((2, 512), 2), ((256, 8192), 1)])
+                                         ((2, 512), 2), ((256, 8192), 1), ((2048, 256), 2), ((512, 4096), 1),
+                                         ((4096, 256), 1)])
 def test_2d_vs_oracle(shape, batch):
     nx, ny = shape
     x = fp16_input((batch, nx, ny), seed=nx * 7 + ny)
