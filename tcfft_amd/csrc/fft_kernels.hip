@@ -42,9 +42,16 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
     const int c16 = lane & 15, g = lane >> 4;
     const float sg = sign < 0 ? 1.f : -1.f;
 
-    // table loads first, then the data (see Dit4096Tw)
+    // table loads first, then the data (see Dit4096Tw); for 1 < T < 16 also
+    // the radix-T twiddle base W_{256T}^c of this lane's four positions
+    // c = 16*c16 + 4*g + i (its powers W_{256T}^{c*d2} follow by recurrence)
     Dit4096Tw tw;
     dit4096_tw_load(tw, lane, w4096);
+    float2 t_base[4];
+    if constexpr (T > 1 && T < 16) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t_base[i] = w4096[((16u * c16 + 4u * g + i) * (16u / T)) & 4095u];
+    }
     __builtin_amdgcn_sched_barrier(0);
     u32 d[NT][4];
 #pragma unroll
@@ -91,7 +98,9 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
     } else if constexpr (T < 16) {
         // Stage 1 on each tile of a line, then radix-T across the line's tiles:
         // X[c + 256*k2] = sum_d2 W_{256T}^{c*d2} Y_d2[c] W_T^{d2*k2}.
-        constexpr int STRIDE = 16 / T;  // W_{256T}^m = W_4096^{m*STRIDE}
+        cf base[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) base[i] = cf{t_base[i].x, sg * t_base[i].y};
 #pragma unroll
         for (int gi = 0; gi < NT / T; ++gi) {
             cf acc[T][4];
@@ -104,12 +113,14 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
             }
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const u32 c = 16u * c16 + 4u * g + i;
                 cf v[T];
                 v[0] = acc[0][i];
+                cf p = base[i];  // W_{256T}^{c*tt}
 #pragma unroll
-                for (int tt = 1; tt < T; ++tt)
-                    v[tt] = cmul(acc[tt][i], twiddle(w4096, ((c * tt) * STRIDE) & 4095u, sg));
+                for (int tt = 1; tt < T; ++tt) {
+                    v[tt] = cmul(acc[tt][i], p);
+                    if (tt + 1 < T) p = cmul(p, base[i]);
+                }
                 small_dft<T>(v, sign);
 #pragma unroll
                 for (int tt = 0; tt < T; ++tt) acc[tt][i] = v[tt];
@@ -153,7 +164,8 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows(u32* __restrict_
                                               const float2* __restrict__ w4096, EpiParams epi,
                                               StartSched sched) {
     const int lane = threadIdx.x & 63;
-    const u32 wave = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // wave index made scalar so that tile addresses are SGPR bases
+    const u32 wave = blockIdx.x * (blockDim.x >> 6) + (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const u32 tile0 = wave * 16u;
     if (tile0 >= n_tiles) return;
     start_delay(sched);
