@@ -189,6 +189,13 @@ def bench_plan1d(torch, tcfft_amd, n, batch, reps, rng):
     ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps)
     out = summary(ms, n, batch, 8.0 * n * batch)
     out.update({"rel_l2_vs_fp64": err, "passes": plan.num_passes, "radices": plan.radices})
+    # natural-order input (tcfft_exec_natural, out of place) on the same data
+    nat = rotation(torch, (batch, n, 2), torch.float16, torch.from_numpy(np.ascontiguousarray(x)))
+    dst = torch.empty_like(nat[0])
+    out["ms_natural_order"] = round(graph_time(
+        torch, lambda i, st: plan.exec_natural(nat[i % len(nat)].data_ptr(), dst.data_ptr(), tcfft_amd.FORWARD, st),
+        reps), 5)
+    del nat
     plan.destroy()
     return out
 
@@ -207,6 +214,12 @@ def bench_plan2d(torch, tcfft_amd, nx, ny, batch, reps, rng):
     out = summary(ms, nx * ny, batch, 8.0 * nx * ny * batch)
     out.update({"rel_l2_vs_fp64": err, "passes": plan.num_passes, "radices_x": plan.radices_x,
                 "radices_y": plan.radices_y})
+    nat = rotation(torch, (batch, nx, ny, 2), torch.float16, torch.from_numpy(np.ascontiguousarray(x)))
+    dst = torch.empty_like(nat[0])
+    out["ms_natural_order"] = round(graph_time(
+        torch, lambda i, st: plan.exec_natural(nat[i % len(nat)].data_ptr(), dst.data_ptr(), tcfft_amd.FORWARD, st),
+        reps), 5)
+    del nat
     plan.destroy()
     return out
 

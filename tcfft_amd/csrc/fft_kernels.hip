@@ -20,6 +20,7 @@
 #include "fft_kernels.hpp"
 
 #include <map>
+#include <type_traits>
 #include <mutex>
 
 namespace tcfft {
@@ -33,9 +34,13 @@ namespace tcfft {
 // below 16 to bound register use); tile t lives at tile_ptr(t) (already
 // offset by this lane's 16*c16 + 4*g); tiles t >= ntl are absent.  tile0 is
 // the global tile index of tile 0 (used by the epilogue only).
-template <int T, bool EPI, int SIGN, int NT = 16, class TilePtr>
+// The data loads are load(d) when a loader is given (natural-order input),
+// else 16-byte loads of tile_ptr(t).
+struct DefaultRowsLoad {};
+
+template <int T, bool EPI, int SIGN, int NT = 16, class TilePtr, class Load = DefaultRowsLoad>
 __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, const float2* __restrict__ w4096,
-                                          const EpiParams& epi) {
+                                          const EpiParams& epi, Load load = {}) {
     static_assert(NT % T == 0 && NT <= 16 && (T < 16 || NT == 16), "tile count per call");
     constexpr int sign = SIGN;
     const int lane = lane_id();
@@ -54,10 +59,14 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
     }
     __builtin_amdgcn_sched_barrier(0);
     u32 d[NT][4];
+    if constexpr (std::is_same_v<Load, DefaultRowsLoad>) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-        if ((u32)t < ntl) load4_nt(d[t], tile_ptr(t));
-        else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
+        for (int t = 0; t < NT; ++t) {
+            if ((u32)t < ntl) load4_nt(d[t], tile_ptr(t));
+            else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
+        }
+    } else {
+        load(d);
     }
     __builtin_amdgcn_sched_barrier(0);
     const Dft16Op op = make_dft16(lane, sign);
@@ -171,6 +180,88 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows(u32* __restrict_
     start_delay(sched);
     u32* const base = buf + (size_t)tile0 * 256 + 16 * (lane & 15) + 4 * (lane >> 4);
     rows_body<T, EPI, SIGN>([&](int t) { return base + t * 256; }, min(16u, n_tiles - tile0), tile0, w4096, epi);
+}
+
+// ----------------------------------------------------------------------------
+// k_rows_nat<T>: the first pass of k_rows<T> on natural-order input (the
+// digit reversal of the caller's gen_rev done in the load), in -> out, in
+// place allowed.  Each wave loads its 16 tiles (whole lines) with 16-byte
+// loads, stores them to its own 16 KiB of LDS and reads them back in the
+// digit-reversed layout of k_rows: register (t, lane (c16, g), word j) holds
+// position p = 256t + 16c16 + 4g + j of the chunk, i.e. natural index
+//   m = (t/T)*256T + (t mod T) + T*c16 + 16T*(4g + j)
+// of its line.  LDS word of m: m ^ ((m >> 6) & 3) * 0x14 (bits 6-7 into
+// bits 2-3 and 4-5): the 16-byte writes stay whole and conflict-free, the
+// b32 reads are conflict-free for T = 1 and at most 4-way for T >= 4.  Only
+// the wave itself touches its LDS, so no barrier is needed (a wave's LDS
+// operations complete in order).  2D (nx_log2 > 0): line l = img*nx + x of
+// the input is stored at line img*nx + pos(x), pos = the digit reversal over
+// the x radices in reverse order (the inverse of the caller's x reversal),
+// out of place only.
+__device__ __forceinline__ u32 nat_lds_word(u32 m) { return m ^ (((m >> 6) & 3u) * 0x14u); }
+
+__device__ __forceinline__ u32 rev_pos_pow2(u32 x, const RevSpec& s) {
+    u32 digits[12];
+    int nd = 0;
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        if (i < s.n_digits) {
+            const u32 r = (u32)s.radix[i];
+            digits[i] = x & (r - 1u);
+            x >>= __builtin_ctz(r);
+            ++nd;
+        }
+    u32 idx = 0;
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        if (i < s.n_digits) idx = idx * (u32)s.radix[i] + digits[i];
+    (void)nd;
+    return idx;
+}
+
+template <int T, int SIGN>
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* __restrict__ in, u32* __restrict__ out,
+                                                                  u32 n_tiles, const float2* __restrict__ w4096,
+                                                                  u32 nx_log2, RevSpec x_pos) {
+    extern __shared__ u32 lds[];
+    constexpr u32 L = 256u * T;
+    const int lane = threadIdx.x & 63;
+    const u32 wib = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 wave = blockIdx.x * (blockDim.x >> 6) + wib;
+    const u32 tile0 = wave * 16u;
+    if (tile0 >= n_tiles) return;
+    const u32 ntl = min(16u, n_tiles - tile0);
+    u32* const wl = lds + wib * 4096u;
+    const int c16 = lane & 15, g = lane >> 4;
+    auto load = [&](u32 (&d)[16][4]) {
+        const u32* src = in + (size_t)tile0 * 256u + 4u * lane;
+        u32 v[16][4];
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            if ((u32)t < ntl) load4_nt(v[t], src + 256u * t);
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            if ((u32)t < ntl) store4(wl + nat_lds_word(256u * t + 4u * lane), v[t]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const u32 m = (u32)(t / T) * L + (u32)(t % T) + T * (u32)c16 + 16u * T * (4u * g + j);
+                d[t][j] = (u32)t < ntl ? wl[nat_lds_word(m)] : 0u;
+            }
+    };
+    const u32 lane_off = 16u * c16 + 4u * g;
+    auto tile_ptr = [&](int t) -> u32* {
+        const u32 line = (tile0 + (u32)t) / T;
+        u32 dst = line;
+        if (nx_log2) {
+            const u32 x = line & ((1u << nx_log2) - 1u);
+            dst = (line & ~((1u << nx_log2) - 1u)) | rev_pos_pow2(x, x_pos);
+        }
+        return out + (size_t)dst * L + 256u * ((u32)t % T) + lane_off;
+    };
+    rows_body<T, false, SIGN>(tile_ptr, ntl, tile0, w4096, EpiParams{}, load);
 }
 
 // ----------------------------------------------------------------------------
@@ -658,6 +749,35 @@ static hipError_t launch_cols_t(u32* buf, u32 s_log2, u32 outer, int sign, const
     if (epi) hipLaunchKernelGGL((k_cols<T, true>), dim3(blocks), dim3(256), 0, st, buf, s_log2, waves, sign, w4096, *epi);
     else hipLaunchKernelGGL((k_cols<T, false>), dim3(blocks), dim3(256), 0, st, buf, s_log2, waves, sign, w4096, EpiParams{});
     return hipGetLastError();
+}
+
+template <int T, int SIGN>
+static hipError_t launch_rows_nat_t(const u32* in, u32* out, u32 n_tiles, const float2* w4096, u32 nx_log2,
+                                    const RevSpec& x_pos, hipStream_t st) {
+    constexpr size_t lds = 4 * 4096 * sizeof(u32);  // 16 KiB per wave
+    auto k = k_rows_nat<T, SIGN>;
+    const KernelInfo ki = kernel_info(k, 256, lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    const u32 waves = (n_tiles + 15) / 16;
+    hipLaunchKernelGGL(k, dim3((waves + 3) / 4), dim3(256), lds, st, in, out, n_tiles, w4096, nx_log2, x_pos);
+    return hipGetLastError();
+}
+
+hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign, const float2* w4096, u32 nx_log2,
+                           const RevSpec& x_pos, hipStream_t st) {
+    switch (T * 2 + (sign < 0 ? 0 : 1)) {
+    case 2: return launch_rows_nat_t<1, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 3: return launch_rows_nat_t<1, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 4: return launch_rows_nat_t<2, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 5: return launch_rows_nat_t<2, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 8: return launch_rows_nat_t<4, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 9: return launch_rows_nat_t<4, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 16: return launch_rows_nat_t<8, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 17: return launch_rows_nat_t<8, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 32: return launch_rows_nat_t<16, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 33: return launch_rows_nat_t<16, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int W, int SIGN>

@@ -89,6 +89,13 @@ struct RevSpec {
     int n_digits;
     int radix[12];
 };
+// First pass of k_rows<T> on natural-order input, in -> out (in place allowed
+// when nx_log2 = 0).  2D (nx_log2 > 0, out of place): input line img*nx + x is
+// written to line img*nx + pos(x), pos = digit reversal over x_pos's radices
+// (the x radices in reverse order).
+hipError_t launch_rows_nat(int T, const uint32_t* in, uint32_t* out, uint32_t n_tiles, int sign, const float2* w4096,
+                           uint32_t nx_log2, const RevSpec& x_pos, hipStream_t st);
+
 hipError_t launch_gather_rev_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
                                 const RevSpec& spec, hipStream_t st);
 hipError_t launch_gather_rev_2d(const uint32_t* in, uint32_t* out, uint32_t nx_log2, uint32_t ny_log2,

@@ -642,6 +642,24 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
     if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
     DeviceGuard guard(plan->device);
     hipStream_t st = (hipStream_t)stream;
+    const Layout& L0 = plan->lay;
+    const Pass& first = L0.passes[0];
+    // Contiguous-row first pass: the digit reversal is done in that pass's
+    // load (1D in place or not; 2D out of place, its x reversal as a line
+    // permutation of the stores), so natural order costs no extra pass.
+    if (first.kind == 0 && !first.has_epi && (L0.dims == 1 || in != out)) {
+        uint32_t nx_log2 = 0;
+        tcfft::RevSpec x_pos{};
+        if (L0.dims == 2) {
+            nx_log2 = (uint32_t)ilog2((uint32_t)L0.nx);
+            std::vector<int> rx(L0.rad_x.rbegin(), L0.rad_x.rend());
+            x_pos = rev_spec(rx);
+        }
+        hipError_t e = tcfft::launch_rows_nat(first.radix, (const uint32_t*)in, (uint32_t*)out, first.n_tiles,
+                                              direction, plan->w4096, nx_log2, x_pos, st);
+        if (e != hipSuccess) return hip_fail(e);
+        return run_passes(plan, (uint32_t*)out, direction, st, 1);
+    }
     const size_t bytes = plan->total * 4;
     const uint32_t* src = (const uint32_t*)in;
     if (in == out) {

@@ -68,9 +68,9 @@ def run_1d(x, direction=tcfft_amd.FORWARD, mode="dropin", inplace=True):
     return out
 
 
-def run_2d(x, direction=tcfft_amd.FORWARD, mode="dropin"):
+def run_2d(x, direction=tcfft_amd.FORWARD, mode="dropin", inplace=True):
     b, nx, ny = x.shape
-    note(f"[gpu] 2d nx={nx} ny={ny} batch={b} dir={direction} mode={mode}")
+    note(f"[gpu] 2d nx={nx} ny={ny} batch={b} dir={direction} mode={mode} inplace={inplace}")
     plan = tcfft_amd.Plan2D(nx, ny, b)
     if mode == "dropin":
         rx = tcfft_amd.gen_rev(nx, plan.radices_x)
@@ -78,8 +78,9 @@ def run_2d(x, direction=tcfft_amd.FORWARD, mode="dropin"):
         d = to_device(x[:, rx][:, :, ry])
         plan.exec(d.data_ptr(), direction)
     else:
-        d = to_device(x)
-        plan.exec_natural(d.data_ptr(), d.data_ptr(), direction)
+        src = to_device(x)
+        d = src if inplace else torch.empty_like(src)
+        plan.exec_natural(src.data_ptr(), d.data_ptr(), direction)
     torch.cuda.synchronize()
     out = from_device(d)
     plan.destroy()
@@ -138,6 +139,42 @@ def test_natural_order_vs_oracle(n, batch, inplace):
     err = oracle.rel_l2(got, oracle.fft(x))
     note(f"  rel_l2={err:.3e}")
     assert err <= tol_for(n)
+
+
+# lines of up to 4096 points: the digit reversal is done in the first pass's
+# load, so the result must equal the drop-in path's bit for bit
+@pytest.mark.parametrize("n,batch,inplace,direction", [
+    (256, 3, False, tcfft_amd.FORWARD), (256, 37, True, tcfft_amd.INVERSE), (256, 65, False, tcfft_amd.FORWARD),
+    (512, 5, True, tcfft_amd.FORWARD), (1024, 7, False, tcfft_amd.INVERSE), (2048, 9, True, tcfft_amd.FORWARD),
+    (4096, 3, False, tcfft_amd.FORWARD), (4096, 17, True, tcfft_amd.INVERSE)])
+def test_natural_fused_first_pass(n, batch, inplace, direction):
+    x = fp16_input((batch, n), seed=9 * n + batch)
+    got = run_1d(x, direction, mode="natural", inplace=inplace)
+    ref = run_1d(x, direction)
+    assert np.array_equal(got, ref)
+    sign = -1 if direction == tcfft_amd.FORWARD else +1
+    err = oracle.rel_l2(got, oracle.fft(x, sign))
+    note(f"  rel_l2={err:.3e}")
+    assert err <= TOL
+
+
+@pytest.mark.parametrize("shape,batch,direction", [((2, 256), 3, tcfft_amd.FORWARD), ((16, 512), 2, tcfft_amd.INVERSE),
+                                                   ((512, 1024), 1, tcfft_amd.FORWARD),
+                                                   ((1024, 1024), 2, tcfft_amd.INVERSE),
+                                                   ((256, 4096), 1, tcfft_amd.FORWARD),
+                                                   ((4096, 256), 1, tcfft_amd.FORWARD)])
+def test_2d_natural_out_of_place(shape, batch, direction):
+    """2D natural order out of place: the y reversal in the rows pass's load,
+    the x reversal as the line order of its stores; equal to the drop-in."""
+    nx, ny = shape
+    x = fp16_input((batch, nx, ny), seed=nx + 3 * ny)
+    got = run_2d(x, direction, mode="natural", inplace=False)
+    ref = run_2d(x, direction)
+    assert np.array_equal(got, ref)
+    sign = -1 if direction == tcfft_amd.FORWARD else +1
+    err = oracle.rel_l2(got, oracle.fft_2d(x, sign))
+    note(f"  rel_l2={err:.3e}")
+    assert err <= TOL
 
 
 @pytest.mark.parametrize("n", [256, 512, 1024, 4096])
