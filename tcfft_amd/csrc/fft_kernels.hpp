@@ -96,6 +96,17 @@ struct RevSpec {
 hipError_t launch_rows_nat(int T, const uint32_t* in, uint32_t* out, uint32_t n_tiles, int sign, const float2* w4096,
                            uint32_t nx_log2, const RevSpec& x_pos, hipStream_t st);
 
+// Tile of the LDS-tiled 1D digit reversal: a low bits of the position and
+// the b position bits in_bit[0..b-1] that form the low bits of the input
+// index (disjoint, a + b <= 12); free_mask = the remaining position bits.
+struct GatherTile {
+    uint32_t a, b;
+    uint32_t free_mask;
+    uint32_t in_bit[6];
+};
+hipError_t launch_gather_tiled_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
+                                  const RevSpec& spec, const GatherTile& gt, hipStream_t st);
+
 hipError_t launch_gather_rev_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
                                 const RevSpec& spec, hipStream_t st);
 hipError_t launch_gather_rev_2d(const uint32_t* in, uint32_t* out, uint32_t nx_log2, uint32_t ny_log2,

@@ -45,6 +45,53 @@ __global__ __launch_bounds__(256) void k_gather_2d(const uint32_t* __restrict__ 
     out[i] = in[src];
 }
 
+// Tiled 1D digit reversal through LDS.  Position p's bit permutation: bit i
+// of rev(p) is bit pi(i) of p.  A tile of 2^(a+b) elements fixes every bit
+// of p except the low a bits (consecutive positions: the writes) and the b
+// bits pi(0..b-1) (consecutive input indices: the reads), which the host
+// chose disjoint.  Tile element tau: its low a bits are p's low a bits, its
+// high b bits are rev(p)'s low b bits.  LDS word tau ^ ((tau >> a) & (2^a-1))
+// keeps both the column-wise writes and the row-wise reads conflict-free.
+__global__ __launch_bounds__(256) void k_gather_tiled(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                      uint32_t n_log2, RevSpec spec, GatherTile gt) {
+    __shared__ uint32_t tile[4096];
+    const uint32_t elems = 1u << (gt.a + gt.b);
+    const uint32_t tiles_per_signal = 1u << (n_log2 - gt.a - gt.b);
+    const uint32_t sig = blockIdx.x / tiles_per_signal;
+    uint32_t f = blockIdx.x % tiles_per_signal;
+    // deposit the tile index into the free bits of p
+    uint32_t p_block = 0, mask = gt.free_mask;
+    while (mask) {
+        const uint32_t bit = (uint32_t)__builtin_ctz(mask);
+        p_block |= (f & 1u) << bit;
+        f >>= 1;
+        mask &= mask - 1u;
+    }
+    const size_t base = (size_t)sig << n_log2;
+    const uint32_t amask = (1u << gt.a) - 1u, bmask = (1u << gt.b) - 1u;
+    auto pos_of = [&](uint32_t tau) -> uint32_t {
+        uint32_t p = p_block | (tau & amask);
+        const uint32_t hi = tau >> gt.a;
+        for (uint32_t i = 0; i < gt.b; ++i) p |= ((hi >> i) & 1u) << gt.in_bit[i];
+        return p;
+    };
+    auto lds_word = [&](uint32_t tau) { return tau ^ ((tau >> gt.a) & amask); };
+    for (uint32_t e = threadIdx.x; e < elems; e += 256u) {
+        const uint32_t tau = ((e & bmask) << gt.a) | (e >> gt.b);  // consecutive e: consecutive rev(p)
+        tile[lds_word(tau)] = in[base + rev_index(pos_of(tau), spec)];
+    }
+    __syncthreads();
+    for (uint32_t e = threadIdx.x; e < elems; e += 256u) out[base + pos_of(e)] = tile[lds_word(e)];
+}
+
+hipError_t launch_gather_tiled_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
+                                  const RevSpec& spec, const GatherTile& gt, hipStream_t st) {
+    if (gt.a + gt.b > 12 || gt.a + gt.b > n_log2 || gt.b > 6) return hipErrorInvalidValue;
+    const uint32_t blocks = total >> (gt.a + gt.b);
+    hipLaunchKernelGGL(k_gather_tiled, dim3(blocks), dim3(256), 0, st, in, out, n_log2, spec, gt);
+    return hipGetLastError();
+}
+
 hipError_t launch_gather_rev_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
                                 const RevSpec& spec, hipStream_t st) {
     hipLaunchKernelGGL(k_gather_1d, dim3((total + 255) / 256), dim3(256), 0, st, in, out, n_log2, total, spec);

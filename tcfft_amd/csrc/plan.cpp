@@ -340,6 +340,42 @@ tcfft::RevSpec rev_spec(const std::vector<int>& rad) {
     return s;
 }
 
+// Tile of the LDS-tiled digit reversal for the radix list `rad` (powers of
+// two): the largest a, b <= 6 whose bit sets are disjoint (see GatherTile).
+bool gather_tile(const std::vector<int>& rad, tcfft::GatherTile& gt) {
+    std::vector<uint32_t> pi;  // bit i of rev(p) = bit pi[i] of p
+    std::vector<uint32_t> start(rad.size()), width(rad.size());
+    uint32_t k = 0;
+    for (size_t d = 0; d < rad.size(); ++d) {
+        start[d] = k;
+        width[d] = (uint32_t)ilog2((uint32_t)rad[d]);
+        k += width[d];
+    }
+    for (size_t d = rad.size(); d-- > 0;)
+        for (uint32_t b = 0; b < width[d]; ++b) pi.push_back(start[d] + b);
+    if (k < 8) return false;
+    uint32_t best_a = 0, best_b = 0;
+    for (uint32_t a = 6; a >= 1; --a)
+        for (uint32_t b = 6; b >= 1; --b) {
+            bool ok = a + b <= k;
+            for (uint32_t i = 0; i < b && ok; ++i) ok = pi[i] >= a;
+            if (ok && a + b > best_a + best_b) {
+                best_a = a;
+                best_b = b;
+            }
+        }
+    if (best_a + best_b < 8) return false;
+    gt.a = best_a;
+    gt.b = best_b;
+    uint32_t used = (1u << best_a) - 1u;
+    for (uint32_t i = 0; i < best_b; ++i) {
+        gt.in_bit[i] = pi[i];
+        used |= 1u << pi[i];
+    }
+    gt.free_mask = ((k >= 32) ? 0xffffffffu : ((1u << k) - 1u)) & ~used;
+    return true;
+}
+
 struct DeviceGuard {
     int prev = -1;
     explicit DeviceGuard(int dev) {
@@ -677,7 +713,11 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
     }
     const Layout& L = plan->lay;
     hipError_t e;
-    if (L.dims == 1)
+    tcfft::GatherTile gt{};
+    if (L.dims == 1 && gather_tile(L.rad_y, gt))
+        e = tcfft::launch_gather_tiled_1d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.n), (uint32_t)plan->total,
+                                          rev_spec(L.rad_y), gt, st);
+    else if (L.dims == 1)
         e = tcfft::launch_gather_rev_1d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.n), (uint32_t)plan->total,
                                         rev_spec(L.rad_y), st);
     else
