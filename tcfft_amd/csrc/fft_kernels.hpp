@@ -102,7 +102,8 @@ hipError_t launch_rows_nat(int T, const uint32_t* in, uint32_t* out, uint32_t n_
 struct GatherTile {
     uint32_t a, b;
     uint32_t free_mask;
-    uint32_t in_bit[6];
+    uint32_t in_bit[6];      // position bit giving bit i of the input index, i < b
+    uint32_t low_to_rev[6];  // input-index bit of position bit j, j < a
 };
 hipError_t launch_gather_tiled_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
                                   const RevSpec& spec, const GatherTile& gt, hipStream_t st);

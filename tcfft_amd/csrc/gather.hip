@@ -8,19 +8,22 @@
 
 namespace tcfft {
 
+// Radices are powers of two: digits by shifts and masks (an integer division
+// by a runtime value costs dozens of instructions per digit).
 __device__ __forceinline__ uint32_t rev_index(uint32_t p, const RevSpec& s) {
     uint32_t digits[12];
 #pragma unroll
     for (int i = 0; i < 12; ++i) {
         if (i < s.n_digits) {
-            digits[i] = p % (uint32_t)s.radix[i];
-            p /= (uint32_t)s.radix[i];
+            const uint32_t w = (uint32_t)__builtin_ctz((uint32_t)s.radix[i]);
+            digits[i] = p & ((1u << w) - 1u);
+            p >>= w;
         }
     }
     uint32_t idx = 0;
 #pragma unroll
     for (int i = 0; i < 12; ++i)
-        if (i < s.n_digits) idx = idx * (uint32_t)s.radix[i] + digits[i];
+        if (i < s.n_digits) idx = (idx << __builtin_ctz((uint32_t)s.radix[i])) | digits[i];
     return idx;
 }
 
@@ -69,16 +72,28 @@ __global__ __launch_bounds__(256) void k_gather_tiled(const uint32_t* __restrict
     }
     const size_t base = (size_t)sig << n_log2;
     const uint32_t amask = (1u << gt.a) - 1u, bmask = (1u << gt.b) - 1u;
+    // rev is a bit permutation: rev(p) = rev(p_block) | (tau >> a) | the low
+    // a bits of p moved to their places in rev (low_to_rev)
+    const uint32_t rev_block = rev_index(p_block, spec);
     auto pos_of = [&](uint32_t tau) -> uint32_t {
         uint32_t p = p_block | (tau & amask);
         const uint32_t hi = tau >> gt.a;
-        for (uint32_t i = 0; i < gt.b; ++i) p |= ((hi >> i) & 1u) << gt.in_bit[i];
+#pragma unroll
+        for (uint32_t i = 0; i < 6; ++i)
+            if (i < gt.b) p |= ((hi >> i) & 1u) << gt.in_bit[i];
         return p;
+    };
+    auto rev_of = [&](uint32_t tau) -> uint32_t {
+        uint32_t r = rev_block | (tau >> gt.a);
+#pragma unroll
+        for (uint32_t j = 0; j < 6; ++j)
+            if (j < gt.a) r |= ((tau >> j) & 1u) << gt.low_to_rev[j];
+        return r;
     };
     auto lds_word = [&](uint32_t tau) { return tau ^ ((tau >> gt.a) & amask); };
     for (uint32_t e = threadIdx.x; e < elems; e += 256u) {
         const uint32_t tau = ((e & bmask) << gt.a) | (e >> gt.b);  // consecutive e: consecutive rev(p)
-        tile[lds_word(tau)] = in[base + rev_index(pos_of(tau), spec)];
+        tile[lds_word(tau)] = in[base + rev_of(tau)];
     }
     __syncthreads();
     for (uint32_t e = threadIdx.x; e < elems; e += 256u) out[base + pos_of(e)] = tile[lds_word(e)];

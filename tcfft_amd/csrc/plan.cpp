@@ -373,6 +373,9 @@ bool gather_tile(const std::vector<int>& rad, tcfft::GatherTile& gt) {
         used |= 1u << pi[i];
     }
     gt.free_mask = ((k >= 32) ? 0xffffffffu : ((1u << k) - 1u)) & ~used;
+    for (uint32_t j = 0; j < best_a; ++j)
+        for (uint32_t i = 0; i < k; ++i)
+            if (pi[i] == j) gt.low_to_rev[j] = i;
     return true;
 }
 
