@@ -10,8 +10,9 @@
 //
 // DIF for N = 4096*W, n = n1 + 4096*n2, k = k2 + W*k1:
 //   step A: Z_k2[n1] = W_N^{n1*k2} * sum_n2 x[n1 + 4096*n2] W_W^{n2*k2}
-//           (VALU, 4 adjacent columns n1 per lane, coalesced loads), then an
-//           LDS exchange so that wave k2 holds line Z_k2;
+//           (VALU radix-W, 4 adjacent columns n1 per lane, coalesced loads),
+//           then an LDS exchange so that wave k2 holds line Z_k2; the factor
+//           W_N^{n1*k2} is applied inside the DIF's first stage (Dif4096Tw);
 //   step B: Y[k2 + W*k1] = sum_n1 Z_k2[n1] W_4096^{n1*k1}, the in-wave
 //           4096-point DIF below, whose output lands in registers exactly
 //           where the DIT kernels load their input.
@@ -31,13 +32,26 @@ namespace tcfft {
 //            W_4096^{a*(e + 16f)} = W_256^{a*f} * (W_4096^a)^e;
 //   stage 3: B-form over a per tile e -> rows h; post(e, i, v) sees every
 //            FP32 result before it is rounded.
+// Line twiddle of the N = 4096*W transform (W > 1): line k2 of step A is
+// Z_k2[n1] = W_N^{n1*k2} * (radix-W sum).  With n1 = a + 16b + 256c it
+// splits into W_N^{256*k2*c} = W_{16W}^{k2*c}, a column scale of the stage-1
+// operand (contraction index c = 4g + j), and W_N^{k2*a} * W_N^{16*k2*b},
+// which does not depend on c and merges into the stage-1 recurrence: it
+// starts at W_N^{k2*a} and steps by W_256^e * W_N^{16*k2} over the tiles b.
+// So step A multiplies by nothing.
 struct Dif4096Tw {
     float2 step1[4];  // W_256^{4g+i}
     float2 init2[4];  // W_256^{(4g+i)*f}, f = lane%16
     float2 step2[4];  // W_4096^{4g+i}
+    float2 col[4];    // line twiddle: W_{16W}^{k2*(4g+j)}
+    float2 s16;       // W_N^{16*k2} = W_{256W}^{k2}
+    RawQ r0;          // W_N^{k2*a}, a = lane%16 (two-level table of W_N)
 };
 
-__device__ __forceinline__ void dif4096_tw_load(Dif4096Tw& tw, int lane, const float2* __restrict__ w4096) {
+template <int W>
+__device__ __forceinline__ void dif4096_tw_load(Dif4096Tw& tw, int lane, const float2* __restrict__ w4096, u32 k2,
+                                                const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
+                                                u32 wn_lo_bits) {
     const int c16 = lane & 15, g = lane >> 4;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
@@ -46,26 +60,67 @@ __device__ __forceinline__ void dif4096_tw_load(Dif4096Tw& tw, int lane, const f
         tw.init2[i] = w4096[16 * ((a * c16) & 255)];
         tw.step2[i] = w4096[a];
     }
+    if constexpr (W > 1) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tw.col[j] = w4096[((k2 * (4u * g + j)) * (256u / W)) & 4095u];
+        tw.s16 = w4096[(k2 * (16u / W)) & 4095u];
+        const u32 m = (k2 * (u32)c16) & (4096u * W - 1u);
+        tw.r0 = RawQ{wn_hi[m >> wn_lo_bits], wn_lo[m & ((1u << wn_lo_bits) - 1u)]};
+    }
 }
 
-template <int SIGN, class Post>
+// DFT-16 operand whose contraction point n = 4g + j (this lane's g) is scaled
+// by t[j]: D = F * diag(t) * X in the B-form.
+__device__ __forceinline__ Dft16Op make_dft16_colscaled(int lane, int sign, const cf (&t)[4]) {
+    const int f = lane & 15, g = lane >> 4;
+    Dft16Op op;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int m = ((4 * g + j) * f) & 15;
+        const float fr0 = cos16(m);
+        const float fi0 = sign < 0 ? -sin16(m) : sin16(m);
+        const float fr = fr0 * t[j].x - fi0 * t[j].y;
+        const float fi = fr0 * t[j].y + fi0 * t[j].x;
+        op.re[2 * j] = (_Float16)fr;
+        op.re[2 * j + 1] = (_Float16)(-fi);
+        op.im[2 * j] = (_Float16)fi;
+        op.im[2 * j + 1] = (_Float16)fr;
+    }
+    return op;
+}
+
+template <int SIGN, int W, class Post>
 __device__ __forceinline__ void dif4096(u32 (&d)[16][4], const Dif4096Tw& tw, const Dft16Op& op, Post post) {
     constexpr float sg = SIGN < 0 ? 1.f : -1.f;
     cf step[4], rec[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        step[i] = cf{tw.step1[i].x, sg * tw.step1[i].y};
-        rec[i] = step[i];
+    for (int i = 0; i < 4; ++i) step[i] = cf{tw.step1[i].x, sg * tw.step1[i].y};
+    Dft16Op op1 = op;
+    if constexpr (W > 1) {
+        const cf s16 = {tw.s16.x, sg * tw.s16.y};
+        const cf r0 = raw_q<SIGN>(tw.r0);
+        cf t[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) t[j] = cf{tw.col[j].x, sg * tw.col[j].y};
+        op1 = make_dft16_colscaled(lane_id(), SIGN, t);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            step[i] = cmul(step[i], s16);
+            rec[i] = r0;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) rec[i] = step[i];
     }
     // stage 1
 #pragma unroll
     for (int b = 0; b < 16; ++b) {
         f32x4 re, im;
-        dft16_b(d[b], op, re, im);
+        dft16_b(d[b], op1, re, im);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const cf v = {re[i], im[i]};
-            if (b) {
+            if (W > 1 || b) {
                 d[b][i] = pack(cmul(v, rec[i]));
                 if (b < 15) rec[i] = cmul(rec[i], step[i]);
             } else {
@@ -114,27 +169,6 @@ __device__ __forceinline__ u32 dif_zpos(u32 n1) {
     return 4u * ((a ^ (b & 3u)) + 16u * b + 256u * (c >> 2)) + ((c & 3u) ^ ((b >> 2) & 3u));
 }
 
-// Raw table entries of W_N^{(4*lane + q)*k2} for k2 = 1..W-1 (two-level W_N).
-template <int W>
-struct StepATw {
-    RawQ base[4][W > 1 ? W - 1 : 1];
-};
-
-template <int W>
-__device__ __forceinline__ void stepa_tw_load(StepATw<W>& t, int lane, const float2* __restrict__ wn_hi,
-                                              const float2* __restrict__ wn_lo, u32 wn_lo_bits) {
-    if constexpr (W > 1) {
-        constexpr u32 mask = 4096u * W - 1u;
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int k = 1; k < W; ++k) {
-                const u32 m = ((4u * lane + q) * k) & mask;
-                t.base[q][k - 1] = RawQ{wn_hi[m >> wn_lo_bits], wn_lo[m & ((1u << wn_lo_bits) - 1u)]};
-            }
-    }
-}
-
 // Loads of the natural-order signal at src for step A (16 x 16 B per lane).
 template <int W>
 __device__ __forceinline__ void stepa_load(u32 (&v)[16][4], const u32* __restrict__ src, int w, int lane) {
@@ -145,18 +179,11 @@ __device__ __forceinline__ void stepa_load(u32 (&v)[16][4], const u32* __restric
         for (int n2 = 0; n2 < W; ++n2) load4_nt(v[s * W + n2], src + 256u * (S * w + s) + 4u * lane + 4096u * n2);
 }
 
+// Radix-W over the W lines for this wave's columns (no twiddle: the line
+// twiddle is folded into the DIF, see Dif4096Tw), written to LDS line k2.
 template <int W, int SIGN>
-__device__ __forceinline__ void stepa_exchange(u32 (&v)[16][4], const StepATw<W>& tw, const float2* __restrict__ w4096,
-                                               int w, int lane, u32* lds) {
+__device__ __forceinline__ void stepa_exchange(u32 (&v)[16][4], int w, int lane, u32* lds) {
     constexpr int S = 16 / W;
-    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
-    cf base[4][W > 1 ? W - 1 : 1];
-    if constexpr (W > 1) {
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int k = 1; k < W; ++k) base[q][k - 1] = raw_q<SIGN>(tw.base[q][k - 1]);
-    }
 #pragma unroll
     for (int s = 0; s < S; ++s) {
         const u32 m = (u32)(S * w + s);
@@ -166,18 +193,9 @@ __device__ __forceinline__ void stepa_exchange(u32 (&v)[16][4], const StepATw<W>
 #pragma unroll
             for (int n2 = 0; n2 < W; ++n2) x[n2] = unpack(v[s * W + n2][q]);
             if constexpr (W > 1) small_dft<W>(x, SIGN);
-            const u32 n1 = 256u * m + 4u * lane + q;
-            const u32 zp = dif_zpos(n1);
+            const u32 zp = dif_zpos(256u * m + 4u * lane + q);
 #pragma unroll
-            for (int k = 0; k < W; ++k) {
-                cf y = x[k];
-                if (k) {
-                    // W_N^{n1*k} = W_N^{(4*lane + q)*k} * W_{16W}^{m*k}
-                    const float2 sw = w4096[((m * k) & (16u * W - 1u)) * (256u / W)];
-                    y = cmul(y, cmul(base[q][k - 1], cf{sw.x, sg * sw.y}));
-                }
-                lds[4096u * k + zp] = pack(y);
-            }
+            for (int k = 0; k < W; ++k) lds[4096u * k + zp] = pack(x[k]);
         }
     }
 }
@@ -206,18 +224,16 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_dif(const u32* __r
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const int c16 = lane & 15, g = lane >> 4;
     Dif4096Tw dtw;
-    dif4096_tw_load(dtw, lane, w4096);
-    StepATw<W> atw;
-    stepa_tw_load<W>(atw, lane, wn_hi, wn_lo, wn_lo_bits);
+    dif4096_tw_load<W>(dtw, lane, w4096, (u32)w, wn_hi, wn_lo, wn_lo_bits);
     __builtin_amdgcn_sched_barrier(0);
     u32 d[16][4];
     stepa_load<W>(d, in + (size_t)blockIdx.x * N, w, lane);
     __builtin_amdgcn_sched_barrier(0);
-    stepa_exchange<W, SIGN>(d, atw, w4096, w, lane, lds);
+    stepa_exchange<W, SIGN>(d, w, lane, lds);
     __syncthreads();
     stepb_read(d, lds, w, lane);
     const Dft16Op op = make_dft16(lane, SIGN);
-    dif4096<SIGN>(d, dtw, op, [](int, int, cf v) -> cf { return v; });
+    dif4096<SIGN, W>(d, dtw, op, [](int, int, cf v) -> cf { return v; });
     u32* dst = out + (size_t)blockIdx.x * N + 4096u * w + 16u * c16 + 4u * g;
 #pragma unroll
     for (int t = 0; t < 16; ++t) store4_nt(dst + 256 * t, d[t]);
@@ -243,14 +259,12 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_conv(const u32* __
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const int c16 = lane & 15, g = lane >> 4;
     Dif4096Tw dtw;
-    dif4096_tw_load(dtw, lane, w4096);
-    StepATw<W> atw;
-    stepa_tw_load<W>(atw, lane, wn_hi, wn_lo, wn_lo_bits);
+    dif4096_tw_load<W>(dtw, lane, w4096, (u32)w, wn_hi, wn_lo, wn_lo_bits);
     __builtin_amdgcn_sched_barrier(0);
     u32 d[16][4];
     stepa_load<W>(d, x + (size_t)sig * N, w, lane);
     __builtin_amdgcn_sched_barrier(0);
-    stepa_exchange<W, -1>(d, atw, w4096, w, lane, lds);
+    stepa_exchange<W, -1>(d, w, lane, lds);
     __syncthreads();
     stepb_read(d, lds, w, lane);
 
@@ -264,7 +278,7 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_conv(const u32* __
     {
         const Dft16Op op = make_dft16(lane, -1);
         u32 sq[4];
-        dif4096<-1>(d, dtw, op, [&](int t, int i, cf v) -> cf {
+        dif4096<-1, W>(d, dtw, op, [&](int t, int i, cf v) -> cf {
             const cf h = unpack(hq[t & 3][i]);
             if (i == 3 && t + 4 < 16) load4(hq[t & 3], hp + 256 * (t + 4));
             if constexpr (SPEC) {
