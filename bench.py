@@ -268,6 +268,7 @@ def bench_uneven(torch, tcfft_amd, reps):
     host = torch.from_numpy(flat)
     bufs = rotation(torch, tuple(host.shape), torch.float16, host)
     g, o, _ = mp.autotune(bufs[0].data_ptr())
+    delay = mp.schedule()[2]
     for b in bufs:
         b.copy_(host)
     ms_multi = graph_time(torch, lambda i, st: mp.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps)
@@ -307,7 +308,7 @@ def bench_uneven(torch, tcfft_amd, reps):
     return {"ms": round(ms_multi, 5), "gflops": round(flops / (ms_multi * 1e-3) / 1e9, 1),
             "hbm_frac": round(8.0 * total / (ms_multi * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
             "ms_per_size_launches": round(ms_sizes, 5), "speedup_vs_per_size": round(ms_sizes / ms_multi, 3),
-            "signals": n_sig, "elements": total, "tasks": n_tasks, "autotuned_grid": g, "autotuned_order": o,
+            "signals": n_sig, "elements": total, "tasks": n_tasks, "autotuned_grid": g, "autotuned_order": o, "autotuned_start_delay": delay,
             "rel_l2_max_sample": max(errs)}
 
 

@@ -152,8 +152,14 @@ int tcfft_multi_exec(tcfft_multi plan, void *data, int direction, void *stream);
 /* Schedule: grid = persistent workgroups (0 = one per task), order = task
  * order (0 largest sizes first, 1 interleaved, 2 smallest first). */
 int tcfft_multi_set_schedule(tcfft_multi plan, int grid, int order);
-/* Times every (grid, order) candidate on `data` (which it overwrites) and
- * keeps the fastest. */
+/* Start delay of the resident workgroups (0..64; 0 = none): they form 3
+ * groups by their index within an XCD and group k starts k * delay quanta of
+ * 1024 GPU cycles late, so that loads and stores of different groups overlap.
+ * Never changes results. */
+int tcfft_multi_set_start_delay(tcfft_multi plan, int delay);
+int tcfft_multi_get_schedule(tcfft_multi plan, int *grid, int *order, int *delay);
+/* Times every (grid, order, start delay 0..2) candidate on `data` (which it
+ * overwrites) and keeps the fastest. */
 int tcfft_multi_autotune(tcfft_multi plan, void *data, void *stream, int *best_grid, int *best_order,
                          float *best_ms);
 

@@ -39,7 +39,7 @@ EXPORTS = (
     "tcfft_exec", "tcfft_exec_natural", "tcfft_gen_rev", "tcfft_plan_num_passes", "tcfft_exec_passes",
     "tcfft_exec_split", "tcfft_plan_set_option",
     "tcfft_multi_create", "tcfft_multi_destroy", "tcfft_multi_info", "tcfft_multi_exec",
-    "tcfft_multi_set_schedule", "tcfft_multi_autotune",
+    "tcfft_multi_set_schedule", "tcfft_multi_autotune", "tcfft_multi_set_start_delay", "tcfft_multi_get_schedule",
     "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad", "tcfft_plan_autotune",
 )
 OPT_SPLIT_CHUNK = 1
@@ -96,6 +96,8 @@ def lib():
         L.tcfft_multi_exec.argtypes = [_vp, _vp, ctypes.c_int, _vp]
         L.tcfft_multi_set_schedule.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.tcfft_multi_autotune.argtypes = [_vp, _vp, _vp, _i32p, _i32p, ctypes.POINTER(ctypes.c_float)]
+        L.tcfft_multi_set_start_delay.argtypes = [_vp, ctypes.c_int]
+        L.tcfft_multi_get_schedule.argtypes = [_vp, _i32p, _i32p, _i32p]
         _lib = L
     return _lib
 
@@ -292,6 +294,16 @@ class MultiPlan:
         _check(lib().tcfft_multi_autotune(self._h, _vp(data_ptr), _vp(stream), ctypes.byref(g), ctypes.byref(o),
                                           ctypes.byref(ms)), "tcfft_multi_autotune")
         return g.value, o.value, ms.value
+
+    def set_start_delay(self, delay: int):
+        _check(lib().tcfft_multi_set_start_delay(self._h, delay), "tcfft_multi_set_start_delay")
+
+    def schedule(self):
+        """(grid, order, start delay) currently used."""
+        g, o, d = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().tcfft_multi_get_schedule(self._h, ctypes.byref(g), ctypes.byref(o), ctypes.byref(d)),
+               "tcfft_multi_get_schedule")
+        return g.value, o.value, d.value
 
     def destroy(self):
         if self._h.value:

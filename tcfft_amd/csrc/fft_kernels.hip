@@ -932,8 +932,10 @@ __device__ __forceinline__ void multi_big(u32* buf, const MultiTask& tk, const u
 template <int SIGN>
 __global__ __launch_bounds__(1024) void k_multi(u32* __restrict__ buf, const MultiTask* __restrict__ tasks,
                                                 u32 n_tasks, const u32* __restrict__ offsets,
-                                                const float2* __restrict__ w4096, MultiTables tb) {
+                                                const float2* __restrict__ w4096, MultiTables tb,
+                                                StartSched sched) {
     extern __shared__ u32 lds[];
+    start_delay(sched);
     for (u32 ti = blockIdx.x; ti < n_tasks; ti += gridDim.x) {
         const MultiTask tk = tasks[ti];
         switch (tk.size_log2) {
@@ -954,21 +956,23 @@ __global__ __launch_bounds__(1024) void k_multi(u32* __restrict__ buf, const Mul
 
 template <int SIGN>
 static hipError_t launch_multi_t(u32* buf, const MultiTask* tasks, u32 n_tasks, const u32* offsets, u32 grid,
-                                 const float2* w4096, const MultiTables& tb, hipStream_t st) {
+                                 const float2* w4096, const MultiTables& tb, StartSched sched, hipStream_t st) {
     constexpr size_t lds = 16 * 2048 * sizeof(u32);  // 128 KiB
     auto k = k_multi<SIGN>;
     const KernelInfo ki = kernel_info(k, 1024, lds);
     if (ki.lds_status != hipSuccess) return ki.lds_status;
-    hipLaunchKernelGGL(k, dim3(grid), dim3(1024), lds, st, buf, tasks, n_tasks, offsets, w4096, tb);
+    if (sched.groups == 0) sched.delay = 0;
+    if (sched.delay) sched.resident = ki.resident;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1024), lds, st, buf, tasks, n_tasks, offsets, w4096, tb, sched);
     return hipGetLastError();
 }
 
 hipError_t launch_multi(u32* buf, const MultiTask* tasks, u32 n_tasks, const u32* offsets, u32 grid, int sign,
-                        const float2* w4096, const MultiTables& tb, hipStream_t st) {
+                        const float2* w4096, const MultiTables& tb, StartSched sched, hipStream_t st) {
     if (n_tasks == 0) return hipSuccess;
     if (grid == 0 || grid > n_tasks) grid = n_tasks;
-    return sign < 0 ? launch_multi_t<-1>(buf, tasks, n_tasks, offsets, grid, w4096, tb, st)
-                    : launch_multi_t<+1>(buf, tasks, n_tasks, offsets, grid, w4096, tb, st);
+    return sign < 0 ? launch_multi_t<-1>(buf, tasks, n_tasks, offsets, grid, w4096, tb, sched, st)
+                    : launch_multi_t<+1>(buf, tasks, n_tasks, offsets, grid, w4096, tb, sched, st);
 }
 
 }  // namespace tcfft

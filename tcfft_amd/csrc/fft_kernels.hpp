@@ -63,7 +63,8 @@ struct MultiTables {  // two-level W_N tables for N = 2^13 .. 2^16
     const float2* lo[4];
 };
 hipError_t launch_multi(uint32_t* buf, const MultiTask* tasks, uint32_t n_tasks, const uint32_t* offsets,
-                        uint32_t grid, int sign, const float2* w4096, const MultiTables& tb, hipStream_t st);
+                        uint32_t grid, int sign, const float2* w4096, const MultiTables& tb, StartSched sched,
+                        hipStream_t st);
 
 // Natural-order transform of N = 4096*W (W = 1, 2, 4) per signal whose output
 // is the spectrum in the digit-reversed order the DIT kernels consume.

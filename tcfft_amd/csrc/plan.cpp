@@ -776,6 +776,7 @@ struct tcfft_multi_s {
     tcfft::MultiTables tb{};
     uint32_t grid = 256;
     int order = 0;
+    int delay = 0;  // start delay of the resident workgroups (3 groups, quanta of 1024 cycles)
     std::vector<void*> allocs;
 };
 
@@ -922,6 +923,21 @@ int tcfft_multi_info(tcfft_multi plan, int* n_tasks, long long* total_elements) 
     return TCFFT_SUCCESS;
 }
 
+int tcfft_multi_set_start_delay(tcfft_multi plan, int delay) {
+    if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
+    if (delay < 0 || delay > 64) return TCFFT_INVALID_VALUE;
+    plan->delay = delay;
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_multi_get_schedule(tcfft_multi plan, int* grid, int* order, int* delay) {
+    if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
+    if (grid) *grid = (int)plan->grid;
+    if (order) *order = plan->order;
+    if (delay) *delay = plan->delay;
+    return TCFFT_SUCCESS;
+}
+
 int tcfft_multi_set_schedule(tcfft_multi plan, int grid, int order) {
     if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
     if (grid < 0 || order < 0 || order > 2) return TCFFT_INVALID_VALUE;
@@ -941,7 +957,8 @@ int tcfft_multi_exec(tcfft_multi plan, void* data, int direction, void* stream) 
     if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
     DeviceGuard guard(plan->device);
     hipError_t e = tcfft::launch_multi((uint32_t*)data, plan->d_tasks, (uint32_t)plan->tasks.size(), plan->d_offsets,
-                                       plan->grid, direction, plan->w4096, plan->tb, (hipStream_t)stream);
+                                       plan->grid, direction, plan->w4096, plan->tb,
+                                       tcfft::StartSched{(uint32_t)plan->delay, 3u, 0u}, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
     return TCFFT_SUCCESS;
 }
@@ -963,30 +980,34 @@ int tcfft_multi_autotune(tcfft_multi plan, void* data, void* stream, int* best_g
     const int grids[] = {cu / 2, cu, 2 * cu};
     float best = 1e30f;
     int bg = cu, bo = 0, rc = TCFFT_SUCCESS;
-    for (int order = 0; order < 3 && !rc; ++order) {
-        for (int gsz : grids) {
-            rc = tcfft_multi_set_schedule(plan, gsz, order);
-            if (rc) break;
-            for (int i = 0; i < 2 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
-            if (rc) break;
-            (void)hipEventRecord(a, st);
-            for (int i = 0; i < 5 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
-            (void)hipEventRecord(b, st);
-            if (hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
-            float ms = 0.f;
-            (void)hipEventElapsedTime(&ms, a, b);
-            ms /= 5.f;
-            if (!rc && ms < best) {
-                best = ms;
-                bg = gsz;
-                bo = order;
+    int bd = 0;
+    // every (task order, grid, start delay), two warm-up runs and five timed
+    for (int order = 0; order < 3 && !rc; ++order)
+        for (int gsz : grids)
+            for (int delay = 0; delay <= 2 && !rc; ++delay) {
+                rc = tcfft_multi_set_schedule(plan, gsz, order);
+                if (rc) break;
+                plan->delay = delay;
+                for (int i = 0; i < 2 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
+                if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+                for (int i = 0; i < 5 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
+                if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+                if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+                float ms = 0.f;
+                if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+                ms /= 5.f;
+                if (!rc && ms < best) {
+                    best = ms;
+                    bg = gsz;
+                    bo = order;
+                    bd = delay;
+                }
             }
-        }
-    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc) return rc;
     rc = tcfft_multi_set_schedule(plan, bg, bo);
+    plan->delay = bd;
     if (best_grid) *best_grid = bg;
     if (best_order) *best_order = bo;
     if (best_ms) *best_ms = best;

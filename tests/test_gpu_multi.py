@@ -72,9 +72,10 @@ def test_multi_schedules_identical_and_autotune():
     sizes, sigs = make(200, 9)
     plan = tcfft_amd.MultiPlan(sizes)
     ref = None
-    for grid, order in ((0, 0), (64, 1), (256, 2), (7, 0)):
-        note(f"[gpu] multi schedule grid={grid} order={order}")
+    for grid, order, delay in ((0, 0, 0), (64, 1, 0), (256, 2, 0), (7, 0, 0), (256, 0, 2), (512, 1, 5)):
+        note(f"[gpu] multi schedule grid={grid} order={order} delay={delay}")
         plan.set_schedule(grid, order)
+        plan.set_start_delay(delay)
         d = pack_dropin(sizes, sigs)
         plan.exec(d.data_ptr())
         torch.cuda.synchronize()
@@ -85,11 +86,20 @@ def test_multi_schedules_identical_and_autotune():
             np.testing.assert_array_equal(out, ref)
     scratch = pack_dropin(sizes, sigs)
     g, o, ms = plan.autotune(scratch.data_ptr())
-    note(f"  autotune -> grid={g} order={o} {ms:.4f} ms")
+    note(f"  autotune -> grid={g} order={o} delay={plan.schedule()[2]} {ms:.4f} ms")
+    assert plan.schedule()[:2] == (g, o) and 0 <= plan.schedule()[2] <= 2
     d = pack_dropin(sizes, sigs)
     plan.exec(d.data_ptr())
     torch.cuda.synchronize()
     np.testing.assert_array_equal(d.cpu().numpy(), ref)
+    plan.destroy()
+
+
+def test_multi_rejects_bad_start_delay():
+    plan = tcfft_amd.MultiPlan([256, 4096])
+    for bad in (-1, 65):
+        with pytest.raises(tcfft_amd.TcfftError):
+            plan.set_start_delay(bad)
     plan.destroy()
 
 
