@@ -17,8 +17,9 @@ Prints one JSON line:
                FETCH_SIZE / WRITE_SIZE counters when that summary is committed
   cpu_baseline = the FP64 checker (oracle/, a scalar C restatement of the
                reference algorithm) on a bounded sample, 1 host core
-  cpu_fft_f32  = scipy.fft complex64 on all host cores, bounded sample
-               (the single-precision host FFT the GPU path replaces)
+  cpu_fft_f32  = the single-precision host FFT the GPU path replaces, on this
+               process's CPU share, bounded sample: FFTW3 API (libfftw3f,
+               else MKL's wrappers) in a child process, else scipy.fft
   configs    = the other workloads (N=4096 x 1024; N=2^20 x 16; 2D 1024^2 x 8;
                split precision; uneven batch; FFT-conv layer), each with its
                time, GFLOP/s, HBM fraction and rel-L2 vs FP64, and hipFFT's
@@ -30,6 +31,7 @@ import argparse
 import json
 import math
 import os
+import subprocess
 import sys
 import time
 
@@ -65,10 +67,10 @@ def parse():
 # ----------------------------------------------------------------------------
 # timing helpers
 
-def graph_time(torch, launch, reps):
-    """Mean ms per launch(i, stream) over reps, captured once into a hipGraph
-    and replayed between synchronizations (replay() runs on the current
-    stream, so the events are recorded there)."""
+def graph_time(torch, launch, reps, replays=1):
+    """ms per launch(i, stream): reps launches captured once into a hipGraph,
+    replayed `replays` times between synchronizations (replay() runs on the
+    current stream, so the events are recorded there); median over replays."""
     g = torch.cuda.CUDAGraph()
     cs = torch.cuda.Stream()
     cs.wait_stream(torch.cuda.current_stream())
@@ -77,13 +79,16 @@ def graph_time(torch, launch, reps):
             launch(i, cs.cuda_stream)
     g.replay()
     torch.cuda.synchronize()
-    a = torch.cuda.Event(enable_timing=True)
-    b = torch.cuda.Event(enable_timing=True)
-    a.record(torch.cuda.current_stream())
-    g.replay()
-    b.record(torch.cuda.current_stream())
-    torch.cuda.synchronize()
-    return a.elapsed_time(b) / reps
+    times = []
+    for _ in range(replays):
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(torch.cuda.current_stream())
+        g.replay()
+        b.record(torch.cuda.current_stream())
+        torch.cuda.synchronize()
+        times.append(a.elapsed_time(b) / reps)
+    return float(np.median(times))
 
 
 def eager_time(torch, fn, reps):
@@ -110,6 +115,16 @@ def rotation(torch, shape, dtype, init):
 
 def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def ref_error(tested, standard):
+    """The reference harness's error statistic (accuracy.cpp:25-41): per real
+    component min(|(t - s) / s|, 1), summed and divided by n * batch."""
+    t = np.stack([tested.real, tested.imag], -1).ravel()
+    s = np.stack([standard.real, standard.imag], -1).ravel()
+    with np.errstate(divide="ignore", invalid="ignore"):
+        v = np.abs((t - s) / s)
+    return float(np.where(v < 1.0, v, 1.0).sum() / (tested.size))
 
 
 def to_c(t):
@@ -144,33 +159,126 @@ def cpu_baseline(n: int):
         oracle.fft(x)
         reps += 1
         el = time.perf_counter() - t0
-        if el > 10.0 or reps >= 50:
+        if el > 10.0 or reps >= 5000:
             break
     flops = 5.0 * n * math.log2(n) * sig * reps
     return {"value": round(flops / el / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "port",
             "sample": f"{reps} x {sig} signals of N={n} (FP64 DIT restatement, {el:.1f} s)"}
 
 
-def cpu_fft_f32(n: int, batch: int):
-    """scipy.fft complex64 on all host cores, bounded sample of the workload."""
-    import scipy.fft
-    cores = os.cpu_count() or 1
+def cpu_threads() -> int:
+    """Host threads of this process's CPU share (affinity, capped by OMP_NUM_THREADS)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    if omp.isdigit() and int(omp) > 0:
+        n = min(n, int(omp))
+    return max(1, n)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# FFTW3-API libraries for the single-precision host FFT, in order of preference:
+# FFTW itself, then MKL's FFTW3 wrappers; scipy.fft is the last resort.
+FFTW_LIBS = ("libfftw3f.so.3", "libfftw3f.so", "/opt/conda/lib/libmkl_rt.so")
+
+
+def fftw_child(n: int, sig: int, threads: int, seconds: float):
+    """Runs in a child process (the FFTW3 library's threading runtime never
+    meets torch's): fftwf_plan_many_dft over `sig` signals of n, in place,
+    complex64, FFTW_MEASURE, `threads` threads; one JSON line on stdout."""
+    import ctypes
+    lib, name = None, None
+    for cand in FFTW_LIBS:
+        try:
+            L = ctypes.CDLL(cand)
+            getattr(L, "fftwf_plan_many_dft")
+            lib, name = L, cand
+            break
+        except (OSError, AttributeError):
+            continue
+    if lib is None:
+        print(json.dumps({"error": "no FFTW3-API library"}))
+        return
+    vp, ip = ctypes.c_void_p, ctypes.c_int
+    lib.fftwf_plan_many_dft.restype = vp
+    lib.fftwf_plan_many_dft.argtypes = [ip, ctypes.POINTER(ip), ip, vp, vp, ip, ip, vp, vp, ip, ip, ip, ctypes.c_uint]
+    lib.fftwf_execute.argtypes = [vp]
+    lib.fftwf_destroy_plan.argtypes = [vp]
+    if hasattr(lib, "fftwf_init_threads"):
+        lib.fftwf_init_threads()
+        lib.fftwf_plan_with_nthreads.argtypes = [ip]
+        lib.fftwf_plan_with_nthreads(threads)
     rng = np.random.default_rng(0)
-    sig = min(batch, 64)
     x = (rng.uniform(-1, 1, (sig, n)) + 1j * rng.uniform(-1, 1, (sig, n))).astype(np.complex64)
-    scipy.fft.fft(x, axis=-1, workers=cores)  # warm-up (plan, threads)
+    buf = np.empty_like(x)
+    dims = (ip * 1)(n)
+    plan = lib.fftwf_plan_many_dft(1, dims, sig, buf.ctypes.data, None, 1, n, buf.ctypes.data, None, 1, n, -1, 0)
+    if not plan:
+        print(json.dumps({"error": f"{name}: planning failed"}))
+        return
+    buf[:] = x
+    lib.fftwf_execute(plan)
+    err = float(np.linalg.norm(buf[0] - np.fft.fft(x[0].astype(np.complex128)))
+                / np.linalg.norm(np.fft.fft(x[0].astype(np.complex128))))
     reps = 0
     t0 = time.perf_counter()
     while True:
-        scipy.fft.fft(x, axis=-1, workers=cores, overwrite_x=False)
+        lib.fftwf_execute(plan)
         reps += 1
         el = time.perf_counter() - t0
-        if el > 5.0 or reps >= 200:
+        if el > seconds or reps >= 100000:
             break
-    flops = 5.0 * n * math.log2(n) * sig * reps
-    gf = flops / el / 1e9
-    return {"value": round(gf, 2), "unit": "GFLOP/s", "cores": cores, "library": f"scipy.fft {scipy.__version__} "
-            "(pocketfft, complex64)", "sample": f"{reps} x {sig} signals of N={n} ({el:.1f} s)",
+    lib.fftwf_destroy_plan(plan)
+    print(json.dumps({"library": name, "reps": reps, "seconds": el, "rel_l2_first_signal": err}))
+
+
+def cpu_fft_f32(n: int, batch: int):
+    """The single-precision host FFT on this process's CPU share, bounded
+    sample of the workload: an FFTW3-API library (FFTW, else MKL's wrappers)
+    in a child process, else scipy.fft (pocketfft)."""
+    cores = cpu_threads()
+    sig = min(batch, 64)
+    flops_per = 5.0 * n * math.log2(n) * sig
+    res = None
+    try:
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), MKL_NUM_THREADS=str(cores))
+        cp = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-fft-child", str(n), str(sig),
+                             str(cores)], capture_output=True, text=True, timeout=120, env=env)
+        lines = [l for l in cp.stdout.splitlines() if l.startswith("{")]
+        if cp.returncode == 0 and lines:
+            r = json.loads(lines[-1])
+            if "error" not in r and r["rel_l2_first_signal"] < 1e-5:
+                res = r
+    except (subprocess.SubprocessError, OSError, ValueError, KeyError):
+        res = None
+    if res is not None:
+        reps, el, library = res["reps"], res["seconds"], f"{res['library']} (fftwf_plan_many_dft, complex64, in place)"
+    else:
+        import scipy.fft
+        rng = np.random.default_rng(0)
+        x = (rng.uniform(-1, 1, (sig, n)) + 1j * rng.uniform(-1, 1, (sig, n))).astype(np.complex64)
+        scipy.fft.fft(x, axis=-1, workers=cores)  # warm-up (plan, threads)
+        reps = 0
+        t0 = time.perf_counter()
+        while True:
+            scipy.fft.fft(x, axis=-1, workers=cores, overwrite_x=False)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el > 5.0 or reps >= 200:
+                break
+        library = f"scipy.fft {scipy.__version__} (pocketfft, complex64; no FFTW3-API library loaded)"
+    gf = flops_per * reps / el / 1e9
+    return {"value": round(gf, 2), "unit": "GFLOP/s", "cores": cores, "cpu_model": cpu_model(), "library": library,
+            "sample": f"{reps} x {sig} signals of N={n} ({el:.1f} s)",
             "ms_full_batch_est": round(5.0 * n * math.log2(n) * batch / (gf * 1e9) * 1e3, 3)}
 
 
@@ -186,7 +294,7 @@ def bench_plan1d(torch, tcfft_amd, n, batch, reps, rng):
     torch.cuda.synchronize()
     k = min(batch, 4)
     err = rel(to_c(bufs[0][:k]), np.fft.fft(x[:k, :, 0].astype(np.float64) + 1j * x[:k, :, 1], axis=-1))
-    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps)
+    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3)
     out = summary(ms, n, batch, 8.0 * n * batch)
     out.update({"rel_l2_vs_fp64": err, "passes": plan.num_passes, "radices": plan.radices})
     # natural-order input (tcfft_exec_natural, out of place) on the same data
@@ -210,7 +318,7 @@ def bench_plan2d(torch, tcfft_amd, nx, ny, batch, reps, rng):
     plan.exec(bufs[0].data_ptr())
     torch.cuda.synchronize()
     err = rel(to_c(bufs[0][:1]), np.fft.fft2(x[:1, :, :, 0].astype(np.float64) + 1j * x[:1, :, :, 1]))
-    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps)
+    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3)
     out = summary(ms, nx * ny, batch, 8.0 * nx * ny * batch)
     out.update({"rel_l2_vs_fp64": err, "passes": plan.num_passes, "radices_x": plan.radices_x,
                 "radices_y": plan.radices_y})
@@ -244,10 +352,10 @@ def bench_split(torch, tcfft_amd, reps, chunk=None, ms_plain=None):
     ref = np.fft.fft(x[:k].astype(np.complex128), axis=-1)
     errs = [rel(got[i], ref[i]) for i in range(k)]
     ms = graph_time(torch, lambda i, st: plan.exec_split(ins[i % len(ins)].data_ptr(), outs[i % len(ins)].data_ptr(),
-                                                         tcfft_amd.FORWARD, st), reps)
+                                                         tcfft_amd.FORWARD, st), reps, 3)
     if ms_plain is None:
         pb = rotation(torch, (batch, n, 2), torch.float16, torch.zeros(batch, n, 2, dtype=torch.float16))
-        ms_plain = graph_time(torch, lambda i, st: plan.exec(pb[i % len(pb)].data_ptr(), tcfft_amd.FORWARD, st), reps)
+        ms_plain = graph_time(torch, lambda i, st: plan.exec(pb[i % len(pb)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3)
     plan.destroy()
     out = summary(ms, n, batch, 16.0 * n * batch)
     out.update({"rel_l2_max_per_signal": max(errs), "ms_plain_fp16": round(ms_plain, 5),
@@ -271,7 +379,7 @@ def bench_uneven(torch, tcfft_amd, reps):
     delay = mp.schedule()[2]
     for b in bufs:
         b.copy_(host)
-    ms_multi = graph_time(torch, lambda i, st: mp.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps)
+    ms_multi = graph_time(torch, lambda i, st: mp.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3)
     classes = sorted(set(int(v) for v in sizes))
     plans, offs, o2 = [], [], 0
     for n in classes:
@@ -285,7 +393,7 @@ def bench_uneven(torch, tcfft_amd, reps):
         for p, off in zip(plans, offs):
             p.exec(base + 4 * off, tcfft_amd.FORWARD, st)
 
-    ms_sizes = graph_time(torch, per_size, reps)
+    ms_sizes = graph_time(torch, per_size, reps, 3)
     # accuracy on a sample of signals, digit-reversed per signal
     sample = [int(i) for i in rng.choice(n_sig, 16, replace=False)]
     check = torch.from_numpy(flat).cuda()
@@ -423,6 +531,9 @@ def load_traffic(n: int, batch: int):
 
 
 def main():
+    if len(sys.argv) == 5 and sys.argv[1] == "--cpu-fft-child":
+        fftw_child(int(sys.argv[2]), int(sys.argv[3]), int(sys.argv[4]), 5.0)
+        return
     args = parse()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}): every workload here shards on "
@@ -457,7 +568,9 @@ def main():
     plan.exec(bufs[0].data_ptr())
     torch.cuda.synchronize()
     k = min(batch, 8)
-    rel_l2 = rel(to_c(bufs[0][:k]), np.fft.fft(x[:k, :, 0].astype(np.float64) + 1j * x[:k, :, 1], axis=-1))
+    ref64 = np.fft.fft(x[:k, :, 0].astype(np.float64) + 1j * x[:k, :, 1], axis=-1)
+    rel_l2 = rel(to_c(bufs[0][:k]), ref64)
+    ref_err = ref_error(to_c(bufs[0][:k]), ref64)
     bufs[0].copy_(host)
 
     nb = len(bufs)
@@ -474,7 +587,7 @@ def main():
     for p in range(plan.num_passes):
         per_pass.append(graph_time(torch, lambda i, st, p=p: plan.exec_passes(bufs[i % nb].data_ptr(), p, 1,
                                                                               tcfft_amd.FORWARD, st),
-                                   max(20, args.steps)))
+                                   max(20, args.steps), 3))
     dom = int(np.argmax(per_pass))
     alg_bytes = 8.0 * n * batch
     achieved = alg_bytes / (per_pass[dom] * 1e-3) / 1e9
@@ -498,6 +611,7 @@ def main():
                                f"digit-reversed input (tcfftExec)", "n": n, "batch": batch,
                    "passes": plan.num_passes, "radices": plan.radices},
         "rel_l2_vs_fp64": rel_l2,
+        "ref_error_metric": ref_err,
         "hbm_frac_whole_exec": round(alg_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
         "pass_ms": [round(v, 5) for v in per_pass],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
