@@ -52,12 +52,18 @@ __global__ __launch_bounds__(256) void k_gather_2d(const uint32_t* __restrict__ 
 // of rev(p) is bit pi(i) of p.  A tile of 2^(a+b) elements fixes every bit
 // of p except the low a bits (consecutive positions: the writes) and the b
 // bits pi(0..b-1) (consecutive input indices: the reads), which the host
-// chose disjoint.  Tile element tau: its low a bits are p's low a bits, its
-// high b bits are rev(p)'s low b bits.  LDS word tau ^ ((tau >> a) & (2^a-1))
-// keeps both the column-wise writes and the row-wise reads conflict-free.
+// chose disjoint (a >= 2, b >= 2).  Tile element tau: its low a bits are p's
+// low a bits, its high b bits are rev(p)'s low b bits, so
+//   p(tau)   = p_block   | (tau & (2^a-1)) | pos_hi[tau >> a]
+//   rev(tau) = rev_block | (tau >> a)      | rev_lo[tau & (2^a-1)]
+// with the two 64-entry deposit tables built once per workgroup in LDS.
+// Each thread moves 4 consecutive elements with one 16-byte load (4
+// consecutive rev) and one 16-byte store (4 consecutive p).  LDS word
+// tau ^ ((tau >> a) & (2^a-1)) spreads both phases over the banks.
 __global__ __launch_bounds__(256) void k_gather_tiled(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                       uint32_t n_log2, RevSpec spec, GatherTile gt) {
     __shared__ uint32_t tile[4096];
+    __shared__ uint32_t pos_hi[64], rev_lo[64];
     const uint32_t elems = 1u << (gt.a + gt.b);
     const uint32_t tiles_per_signal = 1u << (n_log2 - gt.a - gt.b);
     const uint32_t sig = blockIdx.x / tiles_per_signal;
@@ -70,38 +76,50 @@ __global__ __launch_bounds__(256) void k_gather_tiled(const uint32_t* __restrict
         f >>= 1;
         mask &= mask - 1u;
     }
-    const size_t base = (size_t)sig << n_log2;
-    const uint32_t amask = (1u << gt.a) - 1u, bmask = (1u << gt.b) - 1u;
-    // rev is a bit permutation: rev(p) = rev(p_block) | (tau >> a) | the low
-    // a bits of p moved to their places in rev (low_to_rev)
     const uint32_t rev_block = rev_index(p_block, spec);
-    auto pos_of = [&](uint32_t tau) -> uint32_t {
-        uint32_t p = p_block | (tau & amask);
-        const uint32_t hi = tau >> gt.a;
-#pragma unroll
-        for (uint32_t i = 0; i < 6; ++i)
-            if (i < gt.b) p |= ((hi >> i) & 1u) << gt.in_bit[i];
-        return p;
-    };
-    auto rev_of = [&](uint32_t tau) -> uint32_t {
-        uint32_t r = rev_block | (tau >> gt.a);
-#pragma unroll
-        for (uint32_t j = 0; j < 6; ++j)
-            if (j < gt.a) r |= ((tau >> j) & 1u) << gt.low_to_rev[j];
-        return r;
-    };
-    auto lds_word = [&](uint32_t tau) { return tau ^ ((tau >> gt.a) & amask); };
-    for (uint32_t e = threadIdx.x; e < elems; e += 256u) {
-        const uint32_t tau = ((e & bmask) << gt.a) | (e >> gt.b);  // consecutive e: consecutive rev(p)
-        tile[lds_word(tau)] = in[base + rev_of(tau)];
+    const uint32_t amask = (1u << gt.a) - 1u, bmask = (1u << gt.b) - 1u;
+    {
+        const uint32_t t = threadIdx.x & 63u;
+        uint32_t v = 0;
+        if (threadIdx.x < 64u) {
+            for (uint32_t i = 0; i < gt.b; ++i) v |= ((t >> i) & 1u) << gt.in_bit[i];
+            pos_hi[t] = v;
+        } else if (threadIdx.x < 128u) {
+            for (uint32_t j = 0; j < gt.a; ++j) v |= ((t >> j) & 1u) << gt.low_to_rev[j];
+            rev_lo[t] = v;
+        }
     }
     __syncthreads();
-    for (uint32_t e = threadIdx.x; e < elems; e += 256u) out[base + pos_of(e)] = tile[lds_word(e)];
+    const size_t base = (size_t)sig << n_log2;
+    auto lds_word = [&](uint32_t tau) { return tau ^ ((tau >> gt.a) & amask); };
+    for (uint32_t e = 4u * threadIdx.x; e < elems; e += 1024u) {
+        // e .. e+3: consecutive input indices (the low b bits of rev)
+        const uint32_t hi = e >> gt.b;  // tau's low a bits
+        const uint32_t src = rev_block | (e & bmask) | rev_lo[hi & amask];
+        const uint4 v = *reinterpret_cast<const uint4*>(in + base + src);
+        const uint32_t t0 = ((e & bmask) << gt.a) | hi;
+        tile[lds_word(t0)] = v.x;
+        tile[lds_word(t0 + (1u << gt.a))] = v.y;
+        tile[lds_word(t0 + (2u << gt.a))] = v.z;
+        tile[lds_word(t0 + (3u << gt.a))] = v.w;
+    }
+    __syncthreads();
+    for (uint32_t e = 4u * threadIdx.x; e < elems; e += 1024u) {
+        // tau = e .. e+3: consecutive positions (the low a bits of p)
+        const uint32_t dst = p_block | (e & amask) | pos_hi[e >> gt.a];
+        uint4 v;
+        v.x = tile[lds_word(e)];
+        v.y = tile[lds_word(e + 1u)];
+        v.z = tile[lds_word(e + 2u)];
+        v.w = tile[lds_word(e + 3u)];
+        *reinterpret_cast<uint4*>(out + base + dst) = v;
+    }
 }
 
 hipError_t launch_gather_tiled_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
                                   const RevSpec& spec, const GatherTile& gt, hipStream_t st) {
-    if (gt.a + gt.b > 12 || gt.a + gt.b > n_log2 || gt.b > 6) return hipErrorInvalidValue;
+    if (gt.a + gt.b > 12 || gt.a + gt.b > n_log2 || gt.a > 6 || gt.b > 6 || gt.a < 2 || gt.b < 2)
+        return hipErrorInvalidValue;
     const uint32_t blocks = total >> (gt.a + gt.b);
     hipLaunchKernelGGL(k_gather_tiled, dim3(blocks), dim3(256), 0, st, in, out, n_log2, spec, gt);
     return hipGetLastError();

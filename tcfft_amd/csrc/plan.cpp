@@ -364,7 +364,7 @@ bool gather_tile(const std::vector<int>& rad, tcfft::GatherTile& gt) {
                 best_b = b;
             }
         }
-    if (best_a + best_b < 8) return false;
+    if (best_a + best_b < 8 || best_a < 2 || best_b < 2) return false;
     gt.a = best_a;
     gt.b = best_b;
     uint32_t used = (1u << best_a) - 1u;
