@@ -683,25 +683,8 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
     hipStream_t st = (hipStream_t)stream;
     const Layout& L0 = plan->lay;
     const Pass& first = L0.passes[0];
-    // Contiguous-row first pass: the digit reversal is done in that pass's
-    // load (1D in place or not; 2D out of place, its x reversal as a line
-    // permutation of the stores), so natural order costs no extra pass.
-    if (first.kind == 0 && !first.has_epi && (L0.dims == 1 || in != out)) {
-        uint32_t nx_log2 = 0;
-        tcfft::RevSpec x_pos{};
-        if (L0.dims == 2) {
-            nx_log2 = (uint32_t)ilog2((uint32_t)L0.nx);
-            std::vector<int> rx(L0.rad_x.rbegin(), L0.rad_x.rend());
-            x_pos = rev_spec(rx);
-        }
-        hipError_t e = tcfft::launch_rows_nat(first.radix, (const uint32_t*)in, (uint32_t*)out, first.n_tiles,
-                                              direction, plan->w4096, nx_log2, x_pos, st);
-        if (e != hipSuccess) return hip_fail(e);
-        return run_passes(plan, (uint32_t*)out, direction, st, 1);
-    }
     const size_t bytes = plan->total * 4;
-    const uint32_t* src = (const uint32_t*)in;
-    if (in == out) {
+    auto stage = [&](const uint32_t*& src) -> int {  // copy of the input in the plan's workspace
         if (!plan->workspace) {
             hipError_t e = hipMalloc(&plan->workspace, bytes);
             if (e != hipSuccess) {
@@ -713,6 +696,34 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
         hipError_t e = hipMemcpyAsync(plan->workspace, in, bytes, hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) return hip_fail(e);
         src = (const uint32_t*)plan->workspace;
+        return TCFFT_SUCCESS;
+    };
+    // Contiguous-row first pass: the digit reversal is done in that pass's
+    // load (1D in place or not; 2D with its x reversal as a line permutation
+    // of the stores, from a staged copy when in place), so natural order
+    // costs no extra pass.
+    if (first.kind == 0 && !first.has_epi) {
+        const uint32_t* rsrc = (const uint32_t*)in;
+        if (L0.dims == 2 && in == out) {
+            int rc = stage(rsrc);
+            if (rc) return rc;
+        }
+        uint32_t nx_log2 = 0;
+        tcfft::RevSpec x_pos{};
+        if (L0.dims == 2) {
+            nx_log2 = (uint32_t)ilog2((uint32_t)L0.nx);
+            std::vector<int> rx(L0.rad_x.rbegin(), L0.rad_x.rend());
+            x_pos = rev_spec(rx);
+        }
+        hipError_t e = tcfft::launch_rows_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
+                                              plan->w4096, nx_log2, x_pos, st);
+        if (e != hipSuccess) return hip_fail(e);
+        return run_passes(plan, (uint32_t*)out, direction, st, 1);
+    }
+    const uint32_t* src = (const uint32_t*)in;
+    if (in == out) {
+        int rc = stage(src);
+        if (rc) return rc;
     }
     const Layout& L = plan->lay;
     hipError_t e;
