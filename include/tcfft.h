@@ -55,6 +55,21 @@ int tcfft_plan_create_1d(tcfft_plan *plan, int n, int n_batch);
  * ny in {256,512,1024}). */
 int tcfft_plan_create_2d(tcfft_plan *plan, int nx, int ny, int n_batch);
 
+/* Radix split of a 1D plan.  first_log2 = log2 of the first (contiguous)
+ * pass's length when it does not cover the whole line: 8..12 (a row pass of
+ * 256..4096 points) or 16 (the 65536-point single-pass kernel, the default);
+ * the remaining bits go to one or two strided passes.  The radix list
+ * reported by tcfft_plan_radices follows the split.  No reference
+ * counterpart (tcfft_half.cu:1250-1330 has one fixed split per size). */
+int tcfft_plan_create_1d_split(tcfft_plan *plan, int n, int n_batch, int first_log2);
+/* Builds the candidate splits (first pass of 65536 points or the whole line,
+ * and 4096 points), times each on a scratch buffer of the plan's size (the
+ * call allocates and frees it, on the current device's null stream) and
+ * returns the fastest plan; *first_log2 and *best_ms report the choice. */
+int tcfft_plan_create_1d_tuned(tcfft_plan *plan, int n, int n_batch, int *first_log2, float *best_ms);
+int tcfft_query_1d_split(int n, int n_batch, int first_log2, int *radices, int capacity, int *n_radices,
+                         int *n_passes);
+
 /* The reference never frees a plan (tcfft_half.cu:1331-1344 leaks); this does. */
 int tcfft_plan_destroy(tcfft_plan plan);
 

@@ -41,6 +41,7 @@ EXPORTS = (
     "tcfft_multi_create", "tcfft_multi_destroy", "tcfft_multi_info", "tcfft_multi_exec",
     "tcfft_multi_set_schedule", "tcfft_multi_autotune", "tcfft_multi_set_start_delay", "tcfft_multi_get_schedule",
     "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad", "tcfft_plan_autotune",
+    "tcfft_plan_create_1d_split", "tcfft_plan_create_1d_tuned", "tcfft_query_1d_split",
 )
 OPT_SPLIT_CHUNK = 1
 OPT_BIG_DELAY = 2
@@ -87,6 +88,10 @@ def lib():
         L.tcfft_exec_split.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_plan_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.tcfft_plan_autotune.argtypes = [_vp, _vp, _vp, _i32p, _i32p, ctypes.POINTER(ctypes.c_float)]
+        L.tcfft_plan_create_1d_split.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.tcfft_plan_create_1d_tuned.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _i32p,
+                                                 ctypes.POINTER(ctypes.c_float)]
+        L.tcfft_query_1d_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p]
         L.tcfft_exec_dif.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_conv_exec.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
         L.tcfft_conv_filter_grad.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
@@ -125,6 +130,16 @@ def query_1d(n: int, batch: int = 1):
     nr, npass = ctypes.c_int(), ctypes.c_int()
     _check(lib().tcfft_query_1d(n, batch, rad.ctypes.data_as(_i32p), 32, ctypes.byref(nr), ctypes.byref(npass)),
            f"tcfft_query_1d(n={n}, batch={batch})")
+    return [int(v) for v in rad[: nr.value]], npass.value
+
+
+def query_1d_split(n: int, batch: int, first_log2: int):
+    """Host-only: (radices, n_passes) of the 1D plan whose first pass covers
+    2^first_log2 points (tcfft_plan_create_1d_split)."""
+    rad = np.zeros(32, dtype=np.int32)
+    nr, npass = ctypes.c_int(), ctypes.c_int()
+    _check(lib().tcfft_query_1d_split(n, batch, first_log2, rad.ctypes.data_as(_i32p), 32, ctypes.byref(nr),
+                                      ctypes.byref(npass)), f"tcfft_query_1d_split(n={n}, first_log2={first_log2})")
     return [int(v) for v in rad[: nr.value]], npass.value
 
 
@@ -232,11 +247,24 @@ class Plan1D(_Plan):
 
     dims = 1
 
-    def __init__(self, n: int, batch: int = 1):
+    def __init__(self, n: int, batch: int = 1, first_log2: int | None = None, tuned: bool = False):
+        """first_log2: radix split (length of the first pass, 8..12 or 16);
+        tuned: time the candidate splits and keep the fastest (the plan's
+        radices follow the choice, reported in self.first_log2)."""
         super().__init__()
         self.n, self.batch = n, batch
-        _check(lib().tcfft_plan_create_1d(ctypes.byref(self._h), n, batch),
-               f"tcfft_plan_create_1d(n={n}, batch={batch})")
+        self.first_log2, self.tune_ms = first_log2, None
+        if tuned:
+            fl, ms = ctypes.c_int(), ctypes.c_float()
+            _check(lib().tcfft_plan_create_1d_tuned(ctypes.byref(self._h), n, batch, ctypes.byref(fl),
+                                                    ctypes.byref(ms)), f"tcfft_plan_create_1d_tuned(n={n})")
+            self.first_log2, self.tune_ms = fl.value, ms.value
+        elif first_log2 is not None:
+            _check(lib().tcfft_plan_create_1d_split(ctypes.byref(self._h), n, batch, first_log2),
+                   f"tcfft_plan_create_1d_split(n={n}, first_log2={first_log2})")
+        else:
+            _check(lib().tcfft_plan_create_1d(ctypes.byref(self._h), n, batch),
+                   f"tcfft_plan_create_1d(n={n}, batch={batch})")
         self.radices = self._radices(0)
         self.n_radices = len(self.radices)
 

@@ -114,9 +114,9 @@ void set_epi(Pass& p, uint32_t s_log2_axis, int R_next, uint32_t shift) {
 // pass covers L = min(n, 65536) contiguous elements: one wave per line for
 // L <= 4096 (k_rows), one workgroup of L/4096 waves for larger L (k_big,
 // a single HBM round trip).
-int build_axis_rows(int n, long lines, std::vector<int>& rad, std::vector<Pass>& passes) {
+int build_axis_rows(int n, long lines, std::vector<int>& rad, std::vector<Pass>& passes, int first_log2 = 16) {
     const int nb = ilog2((uint32_t)n);
-    const int L = n < 65536 ? n : 65536;
+    const int L = n < (1 << first_log2) ? n : (1 << first_log2);
     Pass first;
     rad.push_back(16);
     rad.push_back(16);
@@ -146,14 +146,19 @@ int build_axis_rows(int n, long lines, std::vector<int>& rad, std::vector<Pass>&
     return 0;
 }
 
-int build_layout_1d(int n, int batch, Layout& out) {
+// first_log2: length 2^first_log2 of the first (contiguous) pass when it does
+// not cover the whole line.  A first pass followed by strided passes folds
+// their twiddle, which the kernels implement for lines of 256..4096 points
+// (k_rows) and 65536 points (k_big<16>): first_log2 in 8..12 or 16.
+int build_layout_1d(int n, int batch, Layout& out, int first_log2 = 16) {
     if (!is_pow2(n) || n < 256 || n > (1 << 27)) return TCFFT_INVALID_SIZE;
     if (batch < 1 || (long)n * batch >= (1L << 31)) return TCFFT_INVALID_BATCH;
+    if (!((first_log2 >= 8 && first_log2 <= 12) || first_log2 == 16)) return TCFFT_INVALID_VALUE;
     out = Layout{};
     out.dims = 1;
     out.n = n;
     out.batch = batch;
-    build_axis_rows(n, batch, out.rad_y, out.passes);
+    build_axis_rows(n, batch, out.rad_y, out.passes, first_log2);
     return TCFFT_SUCCESS;
 }
 
@@ -421,6 +426,80 @@ int tcfft_plan_create_1d(tcfft_plan* plan, int n, int n_batch) {
     return create_common(plan, lay);
 }
 
+int tcfft_plan_create_1d_split(tcfft_plan* plan, int n, int n_batch, int first_log2) {
+    if (!plan) return TCFFT_INVALID_POINTER;
+    *plan = nullptr;
+    Layout lay;
+    int rc = build_layout_1d(n, n_batch, lay, first_log2);
+    if (rc) return rc;
+    return create_common(plan, lay);
+}
+
+int tcfft_plan_create_1d_tuned(tcfft_plan* plan, int n, int n_batch, int* first_log2, float* best_ms) {
+    if (!plan) return TCFFT_INVALID_POINTER;
+    *plan = nullptr;
+    // candidate first-pass lengths: 65536 (or the whole line) and 4096
+    std::vector<int> cands = {16};
+    if (n > 4096) cands.push_back(12);
+    Layout probe;
+    int rc = build_layout_1d(n, n_batch, probe);
+    if (rc) return rc;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return hip_fail(e);
+    void* buf = nullptr;
+    const size_t bytes = (size_t)n * (size_t)n_batch * 4u;
+    e = hipMalloc(&buf, bytes);
+    if (e != hipSuccess) return hip_fail(e);
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess || hipMemset(buf, 0, bytes) != hipSuccess) {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+        (void)hipFree(buf);
+        return TCFFT_HIP_ERROR;
+    }
+    tcfft_plan best = nullptr;
+    float best_t = 1e30f;
+    int best_c = 16;
+    for (int c : cands) {
+        tcfft_plan p = nullptr;
+        rc = tcfft_plan_create_1d_split(&p, n, n_batch, c);
+        if (rc) break;
+        const int reps = 5;
+        for (int i = 0; i < 2 && !rc; ++i) rc = run_passes(p, (uint32_t*)buf, -1, nullptr);
+        if (!rc && hipEventRecord(a, nullptr) != hipSuccess) rc = TCFFT_HIP_ERROR;
+        for (int i = 0; i < reps && !rc; ++i) rc = run_passes(p, (uint32_t*)buf, -1, nullptr);
+        if (!rc && hipEventRecord(b, nullptr) != hipSuccess) rc = TCFFT_HIP_ERROR;
+        if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+        float ms = 0.f;
+        if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+        if (rc) {
+            (void)tcfft_plan_destroy(p);
+            break;
+        }
+        ms /= reps;
+        if (ms < best_t) {
+            if (best) (void)tcfft_plan_destroy(best);
+            best = p;
+            best_t = ms;
+            best_c = c;
+        } else {
+            (void)tcfft_plan_destroy(p);
+        }
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    (void)hipFree(buf);
+    if (rc) {
+        if (best) (void)tcfft_plan_destroy(best);
+        return rc;
+    }
+    *plan = best;
+    if (first_log2) *first_log2 = best_c;
+    if (best_ms) *best_ms = best_t;
+    return TCFFT_SUCCESS;
+}
+
 int tcfft_plan_create_2d(tcfft_plan* plan, int nx, int ny, int n_batch) {
     if (!plan) return TCFFT_INVALID_POINTER;
     *plan = nullptr;
@@ -460,6 +539,15 @@ int tcfft_plan_radices(tcfft_plan plan, int axis, int* radices, int capacity, in
 int tcfft_query_1d(int n, int n_batch, int* radices, int capacity, int* n_radices, int* n_passes) {
     Layout lay;
     int rc = build_layout_1d(n, n_batch, lay);
+    if (rc) return rc;
+    if (n_passes) *n_passes = (int)lay.passes.size();
+    return copy_radices(lay.rad_y, radices, capacity, n_radices);
+}
+
+int tcfft_query_1d_split(int n, int n_batch, int first_log2, int* radices, int capacity, int* n_radices,
+                         int* n_passes) {
+    Layout lay;
+    int rc = build_layout_1d(n, n_batch, lay, first_log2);
     if (rc) return rc;
     if (n_passes) *n_passes = (int)lay.passes.size();
     return copy_radices(lay.rad_y, radices, capacity, n_radices);

@@ -74,6 +74,29 @@ def test_query_radix_lists_reproduce_the_dft(n):
         assert oracle.rel_l2(got, np.fft.fft(x, axis=-1)) < 1e-12
 
 
+@pytest.mark.parametrize("n", [8192, 1 << 16, 1 << 17, 1 << 20, 1 << 24, 1 << 27])
+@pytest.mark.parametrize("first_log2", [8, 10, 12, 16])
+def test_radix_splits(n, first_log2):
+    """Every radix split of a 1D plan: its list reproduces the DFT through
+    gen_rev (oracle, FP64) and at most 9 radices; a first pass of 2^first_log2
+    points plus strided passes."""
+    rad, npass = tcfft_amd.query_1d_split(n, 1, first_log2)
+    assert int(np.prod(rad)) == n and len(rad) <= 9 and all(r in (2, 4, 8, 16) for r in rad)
+    L = min(n, 1 << first_log2)
+    rest = n.bit_length() - L.bit_length()
+    assert npass == 1 + (0 if rest == 0 else 1 if rest <= 11 else -(-rest // 8))
+    if n <= (1 << 17):
+        x = np.random.default_rng(first_log2).standard_normal((1, n)) + 0j
+        got = oracle.dit(x[:, oracle.gen_rev(n, rad)], rad)
+        assert oracle.rel_l2(got, np.fft.fft(x, axis=-1)) < 1e-12
+
+
+def test_radix_split_rejects_unsupported_first_pass():
+    for bad in (7, 13, 14, 15, 17):
+        with pytest.raises(tcfft_amd.TcfftError):
+            tcfft_amd.query_1d_split(1 << 20, 1, bad)
+
+
 @pytest.mark.parametrize("shape", [(256, 256), (512, 1024), (1024, 1024), (16, 256), (2, 512), (4096, 4096)])
 def test_query_2d_radix_lists(shape):
     nx, ny = shape

@@ -91,3 +91,35 @@ def test_bad_schedule_options():
     with pytest.raises(tcfft_amd.TcfftError):
         plan.autotune(d.data_ptr() + 4)
     plan.destroy()
+
+
+# radix splits: a 4096-point first pass (row kernel with the folded twiddle of
+# the strided passes that follow) against the default, and the tuned plan
+@pytest.mark.parametrize("n,batch", [(8192, 17), (1 << 16, 5), (1 << 17, 3), (1 << 20, 2), (1 << 22, 1)])
+def test_radix_split_parity(n, batch):
+    note(f"[gpu] radix split n={n} batch={batch}")
+    x, d = dropin_input(n, batch, 31 + n)
+    tol = 1e-3 if n <= 65536 else 1.5e-3
+    want = np.fft.fft(x, axis=1)
+    for first_log2 in (12, 10):
+        plan = tcfft_amd.Plan1D(n, batch, first_log2=first_log2)
+        assert plan.radices == tcfft_amd.query_1d_split(n, batch, first_log2)[0]
+        # the caller's digit reversal follows the plan's own radix list
+        xr = x[:, tcfft_amd.gen_rev(n, plan.radices)]
+        t = torch.from_numpy(np.ascontiguousarray(np.stack([xr.real, xr.imag], -1).astype(np.float16))).cuda()
+        plan.exec(t.data_ptr())
+        torch.cuda.synchronize()
+        err = rel_l2(t, want)
+        note(f"  first pass 2^{first_log2}: {plan.num_passes} passes, rel_l2={err:.3e}")
+        assert err <= tol
+        plan.destroy()
+    plan = tcfft_amd.Plan1D(n, batch, tuned=True)
+    note(f"  tuned: first pass 2^{plan.first_log2}, {plan.tune_ms:.4f} ms")
+    assert plan.first_log2 in (12, 16) and plan.tune_ms > 0
+    assert plan.radices == tcfft_amd.query_1d(n, batch)[0]  # both candidates share the default digit list
+    t = d.clone()
+    plan.exec(t.data_ptr())
+    torch.cuda.synchronize()
+    assert rel_l2(t, want) <= tol
+    plan.destroy()
+
