@@ -142,13 +142,16 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * launch form GROUPS groups and group k starts k * DELAY quanta (1024 GPU
  * cycles each) late, so that one group's loads overlap another's stores
  * instead of all workgroups loading, computing and storing in lockstep.
+ * TCFFT_OPT_COLS_DELAY (0..64, default 1): the same start delay (3 groups)
+ * for a strided pass of radix 512..2048.
  * The schedule never changes results. */
-enum tcfft_option { TCFFT_OPT_SPLIT_CHUNK = 1, TCFFT_OPT_BIG_DELAY = 2, TCFFT_OPT_BIG_GROUPS = 3 };
+enum tcfft_option { TCFFT_OPT_SPLIT_CHUNK = 1, TCFFT_OPT_BIG_DELAY = 2, TCFFT_OPT_BIG_GROUPS = 3, TCFFT_OPT_COLS_DELAY = 4 };
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
 /* Times the plan's forward transform under each start schedule candidate
  * (groups, delay) on `data` (which it overwrites with transforms of
- * itself), keeps the fastest in the plan and reports it.  No reference
+ * itself), keeps the fastest in the plan and reports it; then the column
+ * pass delay (0..2) when the plan has a strided pass of radix 512..2048.  No reference
  * counterpart (the reference has one fixed launch per size). */
 int tcfft_plan_autotune(tcfft_plan plan, void *data, void *stream, int *best_groups, int *best_delay,
                         float *best_ms);

@@ -46,6 +46,7 @@ EXPORTS = (
 OPT_SPLIT_CHUNK = 1
 OPT_BIG_DELAY = 2
 OPT_BIG_GROUPS = 3
+OPT_COLS_DELAY = 4
 
 
 class TcfftError(RuntimeError):

@@ -522,10 +522,12 @@ struct ColsBigCfg {
 template <int W, int SIGN>
 __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __restrict__ buf, u32 s_log2,
                                                                      u32 n_groups,
-                                                                     const float2* __restrict__ w4096) {
+                                                                     const float2* __restrict__ w4096,
+                                                                     StartSched sched) {
     using C = ColsBigCfg<W>;
     static_assert(W == 2 || W == 4 || W == 8, "k_cols_big: W = 2, 4, 8");
     extern __shared__ u32 lds[];
+    start_delay(sched);
     constexpr float sg = SIGN < 0 ? 1.f : -1.f;
     const int lane = lane_id();
     const int c16 = lane & 15, g = lane >> 4;
@@ -781,25 +783,29 @@ hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign
 }
 
 template <int W, int SIGN>
-static hipError_t launch_cols_big_t(u32* buf, u32 s_log2, u32 outer, const float2* w4096, hipStream_t st) {
+static hipError_t launch_cols_big_t(u32* buf, u32 s_log2, u32 outer, const float2* w4096, StartSched sched,
+                                    hipStream_t st) {
     const size_t lds = ColsBigCfg<W>::LDS_WORDS * sizeof(u32);
     auto k = k_cols_big<W, SIGN>;
     const KernelInfo ki = kernel_info(k, 64 * W, lds);
     if (ki.lds_status != hipSuccess) return ki.lds_status;
+    if (sched.groups == 0) sched.delay = 0;
+    if (sched.delay) sched.resident = ki.resident;
     const u32 n_groups = outer << (s_log2 - 4u);
-    hipLaunchKernelGGL(k, dim3(n_groups), dim3(64 * W), lds, st, buf, s_log2, n_groups, w4096);
+    hipLaunchKernelGGL(k, dim3(n_groups), dim3(64 * W), lds, st, buf, s_log2, n_groups, w4096, sched);
     return hipGetLastError();
 }
 
-hipError_t launch_cols_big(int W, u32* buf, u32 s_log2, u32 outer, int sign, const float2* w4096, hipStream_t st) {
+hipError_t launch_cols_big(int W, u32* buf, u32 s_log2, u32 outer, int sign, const float2* w4096, StartSched sched,
+                           hipStream_t st) {
     if (s_log2 < 4u || ((size_t)outer << (s_log2 - 4u)) > 0x7fffffffu) return hipErrorInvalidValue;
     switch (W * 2 + (sign < 0 ? 0 : 1)) {
-    case 4: return launch_cols_big_t<2, -1>(buf, s_log2, outer, w4096, st);
-    case 5: return launch_cols_big_t<2, +1>(buf, s_log2, outer, w4096, st);
-    case 8: return launch_cols_big_t<4, -1>(buf, s_log2, outer, w4096, st);
-    case 9: return launch_cols_big_t<4, +1>(buf, s_log2, outer, w4096, st);
-    case 16: return launch_cols_big_t<8, -1>(buf, s_log2, outer, w4096, st);
-    case 17: return launch_cols_big_t<8, +1>(buf, s_log2, outer, w4096, st);
+    case 4: return launch_cols_big_t<2, -1>(buf, s_log2, outer, w4096, sched, st);
+    case 5: return launch_cols_big_t<2, +1>(buf, s_log2, outer, w4096, sched, st);
+    case 8: return launch_cols_big_t<4, -1>(buf, s_log2, outer, w4096, sched, st);
+    case 9: return launch_cols_big_t<4, +1>(buf, s_log2, outer, w4096, sched, st);
+    case 16: return launch_cols_big_t<8, -1>(buf, s_log2, outer, w4096, sched, st);
+    case 17: return launch_cols_big_t<8, +1>(buf, s_log2, outer, w4096, sched, st);
     default: return hipErrorInvalidValue;
     }
 }

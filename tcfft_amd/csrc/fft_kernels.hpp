@@ -44,7 +44,7 @@ hipError_t launch_big(int W, uint32_t* buf, uint32_t n_signals, int sign, const 
 // Last strided pass of its axis with radix R = 256*W (W = 2, 4, 8) at element
 // step 2^s_log2 over `outer` [R][s] blocks, in one HBM round trip.
 hipError_t launch_cols_big(int W, uint32_t* buf, uint32_t s_log2, uint32_t outer, int sign, const float2* w4096,
-                           hipStream_t st);
+                           StartSched sched, hipStream_t st);
 
 // Split-precision N = 65536 transform of FP32 complex data (in != out), as
 // two passes per chunk of `chunk` signals (0 = all at once).

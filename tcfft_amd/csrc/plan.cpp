@@ -204,6 +204,7 @@ struct tcfft_plan_s {
     int cus = 0;          // compute units of the plan's device
     int big_delay = -1;   // start schedule of the first-pass kernels: -1 = default
     int big_groups = 3;
+    int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
 };
 
 namespace {
@@ -331,7 +332,8 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
                                   epi, big_sched(p, ps.radix), st);
         else if (ps.kind == 3)
             e = epi ? hipErrorInvalidValue
-                    : tcfft::launch_cols_big(ps.radix / 256, buf, ps.s_log2, ps.outer, sign, p->w4096, st);
+                    : tcfft::launch_cols_big(ps.radix / 256, buf, ps.s_log2, ps.outer, sign, p->w4096,
+                                             tcfft::StartSched{(uint32_t)p->cols_delay, 3u, 0u}, st);
         else e = tcfft::launch_cols(ps.radix, buf, ps.s_log2, ps.outer, sign, p->w4096, epi, st);
         if (e != hipSuccess) return hip_fail(e);
     }
@@ -681,6 +683,10 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
         if (value < -1 || value > 64) return TCFFT_INVALID_VALUE;
         plan->big_delay = value;
         return TCFFT_SUCCESS;
+    case TCFFT_OPT_COLS_DELAY:
+        if (value < 0 || value > 64) return TCFFT_INVALID_VALUE;
+        plan->cols_delay = value;
+        return TCFFT_SUCCESS;
     case TCFFT_OPT_BIG_GROUPS:
         if (value < 1 || value > 8) return TCFFT_INVALID_VALUE;
         plan->big_groups = value;
@@ -740,9 +746,44 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
         if (c.ms < best->ms) best = &c;
     plan->big_groups = best->groups;
     plan->big_delay = best->delay;
+    float best_t = best->ms;
     if (best_groups) *best_groups = best->groups;
     if (best_delay) *best_delay = best->delay;
-    if (best_ms) *best_ms = best->ms;
+    // then the start delay of a radix-256*W column pass, if the plan has one
+    bool has_cols_big = false;
+    for (const Pass& ps : plan->lay.passes) has_cols_big |= ps.kind == 3;
+    if (has_cols_big) {
+        const int saved = plan->cols_delay;
+        int best_cd = saved;
+        float best_cd_t = 1e30f;
+        hipEvent_t c0, c1;
+        if (hipEventCreate(&c0) != hipSuccess) return TCFFT_HIP_ERROR;
+        if (hipEventCreate(&c1) != hipSuccess) {
+            (void)hipEventDestroy(c0);
+            return TCFFT_HIP_ERROR;
+        }
+        for (int round = 0; round < 2 && !rc; ++round)
+            for (int cd = 0; cd <= 2 && !rc; ++cd) {
+                plan->cols_delay = cd;
+                for (int i = 0; i < 2 && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
+                if (!rc && hipEventRecord(c0, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+                for (int i = 0; i < reps && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
+                if (!rc && hipEventRecord(c1, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+                if (!rc && hipEventSynchronize(c1) != hipSuccess) rc = TCFFT_HIP_ERROR;
+                float ms = 0.f;
+                if (!rc && hipEventElapsedTime(&ms, c0, c1) != hipSuccess) rc = TCFFT_HIP_ERROR;
+                if (!rc && ms / reps < best_cd_t) {
+                    best_cd_t = ms / reps;
+                    best_cd = cd;
+                }
+            }
+        (void)hipEventDestroy(c0);
+        (void)hipEventDestroy(c1);
+        plan->cols_delay = rc ? saved : best_cd;
+        if (rc) return rc;
+        best_t = std::min(best_t, best_cd_t);
+    }
+    if (best_ms) *best_ms = best_t;
     return TCFFT_SUCCESS;
 }
 

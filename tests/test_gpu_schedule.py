@@ -81,6 +81,36 @@ def test_autotune_then_parity(n, batch):
     plan.destroy()
 
 
+@pytest.mark.parametrize("shape,batch", [((1024, 1024), 2), ((2048, 256), 3), ((512, 512), 4)])
+def test_column_pass_delay_bit_identical(shape, batch):
+    """Start delay of the radix-256*W column pass (2D nx = 512..2048): every
+    value gives the same result bit for bit; the 2D autotuner keeps parity."""
+    nx, ny = shape
+    note(f"[gpu] column pass delay {nx}x{ny}x{batch}")
+    rng = np.random.default_rng(nx + ny)
+    x = torch.from_numpy(rng.uniform(-1, 1, (batch, nx, ny, 2)).astype(np.float16)).cuda()
+    plan = tcfft_amd.Plan2D(nx, ny, batch)
+    ref = x.clone()
+    plan.set_option(tcfft_amd.OPT_COLS_DELAY, 0)
+    plan.exec(ref.data_ptr())
+    for d in (1, 2, 5, 64):
+        plan.set_option(tcfft_amd.OPT_COLS_DELAY, d)
+        t = x.clone()
+        plan.exec(t.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(t, ref), d
+    scratch = x.clone()
+    plan.autotune(scratch.data_ptr())
+    t = x.clone()
+    plan.exec(t.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(t, ref)
+    for bad in (-1, 65):
+        with pytest.raises(tcfft_amd.TcfftError):
+            plan.set_option(tcfft_amd.OPT_COLS_DELAY, bad)
+    plan.destroy()
+
+
 def test_bad_schedule_options():
     plan = tcfft_amd.Plan1D(65536, 1)
     for opt, val in [(tcfft_amd.OPT_BIG_DELAY, -2), (tcfft_amd.OPT_BIG_DELAY, 65), (tcfft_amd.OPT_BIG_GROUPS, 0),
