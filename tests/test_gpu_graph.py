@@ -1,0 +1,146 @@
+"""hipGraph capture of the entry points that allocate nothing: the drop-in
+exec (1D and 2D), out-of-place natural order, the uneven batch, split
+precision and the fused convolution.  Each graph is replayed three times with
+eager work (input restore, an eager transform of another buffer) between the
+replays, and every replay must equal the eager result bit for bit."""
+import sys
+
+import numpy as np
+import pytest
+
+import tcfft_amd
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+
+
+def note(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def fp16(rng, shape):
+    return torch.from_numpy(rng.uniform(-1, 1, shape).astype(np.float16)).cuda()
+
+
+def replay_check(launch, reset, out, want, other):
+    """Capture launch(stream) once; then three times: reset inputs eagerly,
+    run an eager transform of `other`, replay, compare `out` with `want`."""
+    g = torch.cuda.CUDAGraph()
+    cs = torch.cuda.Stream()
+    reset()
+    cs.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.graph(g, stream=cs):
+        launch(cs.cuda_stream)
+    torch.cuda.synchronize()
+    for r in range(3):
+        reset()
+        other()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(out, want), f"replay {r}"
+
+
+def test_graph_dropin_1d_and_2d():
+    rng = np.random.default_rng(1)
+    note("[gpu] graph: drop-in 1D 2^16 x 8 and 2D 512 x 1024 x 2")
+    p1 = tcfft_amd.Plan1D(1 << 16, 8)
+    p2 = tcfft_amd.Plan2D(512, 1024, 2)
+    x1, x2 = fp16(rng, (8, 1 << 16, 2)), fp16(rng, (2, 512, 1024, 2))
+    w1, w2 = x1.clone(), x2.clone()
+    p1.exec(w1.data_ptr())
+    p2.exec(w2.data_ptr())
+    torch.cuda.synchronize()
+    b1, b2 = x1.clone(), x2.clone()
+    scratch = x1.clone()
+
+    def reset():
+        b1.copy_(x1)
+        b2.copy_(x2)
+
+    def launch(st):
+        p1.exec(b1.data_ptr(), tcfft_amd.FORWARD, st)
+        p2.exec(b2.data_ptr(), tcfft_amd.FORWARD, st)
+
+    def other():
+        scratch.copy_(x1)
+        p1.exec(scratch.data_ptr())
+
+    replay_check(launch, reset, b1, w1, other)
+    assert torch.equal(b2, w2)
+    p1.destroy()
+    p2.destroy()
+
+
+@pytest.mark.parametrize("n,batch", [(4096, 16), (1 << 16, 4)])
+def test_graph_natural_out_of_place(n, batch):
+    note(f"[gpu] graph: natural order n={n} batch={batch}")
+    rng = np.random.default_rng(n)
+    plan = tcfft_amd.Plan1D(n, batch)
+    x = fp16(rng, (batch, n, 2))
+    want = torch.empty_like(x)
+    plan.exec_natural(x.data_ptr(), want.data_ptr())
+    torch.cuda.synchronize()
+    src, dst = x.clone(), torch.empty_like(x)
+
+    def reset():
+        src.copy_(x)
+        dst.zero_()
+
+    replay_check(lambda st: plan.exec_natural(src.data_ptr(), dst.data_ptr(), tcfft_amd.FORWARD, st), reset, dst,
+                 want, lambda: plan.exec(torch.empty_like(x).copy_(x).data_ptr()))
+    plan.destroy()
+
+
+def test_graph_uneven_batch():
+    note("[gpu] graph: uneven batch")
+    rng = np.random.default_rng(5)
+    sizes = (2 ** rng.integers(8, 17, 120)).astype(np.int32)
+    mp = tcfft_amd.MultiPlan(sizes)
+    total = int(sizes.sum())
+    x = fp16(rng, (total, 2))
+    want = x.clone()
+    mp.exec(want.data_ptr())
+    torch.cuda.synchronize()
+    b = x.clone()
+    scratch = x.clone()
+    replay_check(lambda st: mp.exec(b.data_ptr(), tcfft_amd.FORWARD, st), lambda: b.copy_(x), b, want,
+                 lambda: mp.exec(scratch.copy_(x).data_ptr()))
+    mp.destroy()
+
+
+def test_graph_split_precision():
+    note("[gpu] graph: split precision 2^16 x 8")
+    rng = np.random.default_rng(7)
+    plan = tcfft_amd.Plan1D(1 << 16, 8)
+    x = torch.from_numpy(rng.standard_normal((8, 1 << 16, 2)).astype(np.float32)).cuda()
+    want = torch.empty_like(x)
+    plan.exec_split(x.data_ptr(), want.data_ptr())
+    torch.cuda.synchronize()
+    out = torch.empty_like(x)
+    replay_check(lambda st: plan.exec_split(x.data_ptr(), out.data_ptr(), tcfft_amd.FORWARD, st),
+                 lambda: out.zero_(), out, want, lambda: plan.exec_split(x.data_ptr(), torch.empty_like(x).data_ptr()))
+    plan.destroy()
+
+
+def test_graph_fused_convolution():
+    note("[gpu] graph: fused convolution n=8192, 4 x 8 channels")
+    rng = np.random.default_rng(9)
+    n, ch, b = 8192, 8, 4
+    plan = tcfft_amd.Plan1D(n, b * ch)
+    x = fp16(rng, (b, ch, n, 2))
+    h = (fp16(rng, (ch, n, 2)) * 0.02).contiguous()
+    want, spec_w = torch.empty_like(x), torch.empty_like(x)
+    plan.conv_exec(ch, x.data_ptr(), h.data_ptr(), want.data_ptr(), False, spec_w.data_ptr())
+    torch.cuda.synchronize()
+    y, spec = torch.empty_like(x), torch.empty_like(x)
+
+    def reset():
+        y.zero_()
+        spec.zero_()
+
+    replay_check(lambda st: plan.conv_exec(ch, x.data_ptr(), h.data_ptr(), y.data_ptr(), False, spec.data_ptr(), st),
+                 reset, y, want, lambda: plan.exec_dif(x.data_ptr(), torch.empty_like(x).data_ptr()))
+    assert torch.equal(spec, spec_w)
+    plan.destroy()
