@@ -93,8 +93,16 @@ int tcfft_query_2d(int nx, int ny, int n_batch, int *radices_x, int *n_radices_x
  * `stream`. */
 int tcfft_exec(tcfft_plan plan, void *data, int direction, void *stream);
 
-/* Natural-order input and output.  in == out is allowed (uses a plan-owned
- * device workspace of the buffer's size, allocated on first use). */
+/* Natural-order input and output; the result equals tcfft_exec on the input
+ * permuted by the caller-side gen_rev (tcfft_doit_half.cpp:40-46), bit for
+ * bit.  The digit reversal runs in the first pass's load for 1D lines of up
+ * to 4096 points and 2D rows of up to 4096 points, else as one extra tiled
+ * reversal pass.  in == out is allowed; in and out must not otherwise
+ * overlap.  The 2D in-place case and the 1D in-place case above 4096 points
+ * stage the input in a plan-owned device workspace of the buffer's size
+ * (allocated on first use), so concurrent in-place calls on one plan from
+ * different streams race on it; all other cases allocate nothing and can be
+ * captured into a hipGraph. */
 int tcfft_exec_natural(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
 
 /* Split precision (no counterpart in the reference; config "FP16 hi+lo of
