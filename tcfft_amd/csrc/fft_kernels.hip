@@ -670,6 +670,80 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big(u32* __restric
 }
 
 // ----------------------------------------------------------------------------
+// k_big_nat<W>: k_big<W> (W = 2, 4, 8) on natural-order input, in -> out (in
+// place allowed: a workgroup reads its whole signal before its first store).
+// Each wave loads its natural 4096-element chunk with 16-byte loads and stores
+// it to LDS (the signal, W*16 KiB, fits at once); after a barrier wave w reads
+// the chunk k_big loads in the drop-in order: register (t, lane (c16, g),
+// word j) = position 4096w + 256t + 16c16 + 4g + j, natural index
+// w + W*(t + 16*c16 + 256*(4g + j)).  The LDS word XORs two 2-bit fields of
+// the index into bits 2-3 and 4-5 (NatBigSwz): the 16-byte writes stay whole
+// and conflict-free, the b32 reads drop from 32..64-way to 4-way conflicts.
+// 2D (nx_log2 > 0, out of place): line img*nx + x goes to img*nx + pos(x).
+template <int W>
+struct NatBigSwz;
+template <>
+struct NatBigSwz<2> {
+    static constexpr u32 A = 7, B = 6;
+};
+template <>
+struct NatBigSwz<4> {
+    static constexpr u32 A = 6, B = 8;
+};
+template <>
+struct NatBigSwz<8> {
+    static constexpr u32 A = 7, B = 9;
+};
+
+template <int W>
+__device__ __forceinline__ u32 nat_big_word(u32 m) {
+    return m ^ (((m >> NatBigSwz<W>::A) & 3u) << 2) ^ (((m >> NatBigSwz<W>::B) & 3u) << 4);
+}
+
+template <int W, int SIGN>
+__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
+    const u32* __restrict__ in, u32* __restrict__ out, const float2* __restrict__ w4096,
+    const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo, u32 wn_lo_bits, u32 nx_log2, RevSpec x_pos,
+    StartSched sched) {
+    extern __shared__ u32 lds[];
+    constexpr u32 N = 4096u * W;
+    start_delay(sched);
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+    const u32 w = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 sig = blockIdx.x;
+    BigTw bt;
+    big_tw_load<W, false>(bt, lane, (int)w, sig, w4096, wn_hi, wn_lo, wn_lo_bits, EpiParams{});
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        u32 v[16][4];
+        const u32* src = in + (size_t)sig * N + 4096u * w + 4u * lane;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) load4_nt(v[t], src + 256u * t);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) store4(lds + nat_big_word<W>(4096u * w + 256u * t + 4u * lane), v[t]);
+    }
+    __syncthreads();
+    u32 d[16][4];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            d[t][j] = lds[nat_big_word<W>(w + W * ((u32)t + 16u * c16 + 256u * (4u * g + j)))];
+    __syncthreads();  // every wave has read its chunk before the exchange reuses the LDS
+    u32 dst = sig;
+    if (nx_log2) {
+        const u32 x = sig & ((1u << nx_log2) - 1u);
+        dst = (sig & ~((1u << nx_log2) - 1u)) | rev_pos_pow2(x, x_pos);
+    }
+    const Dft16Op op = make_dft16(lane, SIGN);
+    u32 e[16][4];
+    big_phase1<SIGN>(d, e, bt, op);
+    big_phase2<W, BigCfg<W>::ROUNDS, false, SIGN>(e, (int)w, lds, out + (size_t)dst * N, bt, op, true);
+}
+
+// ----------------------------------------------------------------------------
 // Host-side launchers (called by the pass driver).
 // One-time set-up of a kernel, cached by the kernel's address (every
 // template instance of a kernel has the same pointer type, so a static per
@@ -778,6 +852,36 @@ hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign
     case 17: return launch_rows_nat_t<8, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
     case 32: return launch_rows_nat_t<16, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
     case 33: return launch_rows_nat_t<16, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int W, int SIGN>
+static hipError_t launch_big_nat_t(const u32* in, u32* out, u32 n_signals, const float2* w4096, const float2* wn_hi,
+                                   const float2* wn_lo, u32 lo_bits, u32 nx_log2, const RevSpec& x_pos,
+                                   StartSched sched, hipStream_t st) {
+    const size_t lds = 4096u * W * sizeof(u32);  // the whole signal (>= the exchange's BigCfg<W>::LDS_WORDS)
+    static_assert(4096 * W >= BigCfg<W>::LDS_WORDS, "LDS covers the exchange");
+    auto k = k_big_nat<W, SIGN>;
+    const KernelInfo ki = kernel_info(k, 64 * W, lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    if (sched.groups == 0) sched.delay = 0;
+    if (sched.delay) sched.resident = ki.resident;
+    hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, in, out, w4096, wn_hi, wn_lo, lo_bits, nx_log2,
+                       x_pos, sched);
+    return hipGetLastError();
+}
+
+hipError_t launch_big_nat(int W, const u32* in, u32* out, u32 n_signals, int sign, const float2* w4096,
+                          const float2* wn_hi, const float2* wn_lo, u32 lo_bits, u32 nx_log2, const RevSpec& x_pos,
+                          StartSched sched, hipStream_t st) {
+    switch (W * 2 + (sign < 0 ? 0 : 1)) {
+    case 4: return launch_big_nat_t<2, -1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
+    case 5: return launch_big_nat_t<2, +1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
+    case 8: return launch_big_nat_t<4, -1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
+    case 9: return launch_big_nat_t<4, +1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
+    case 16: return launch_big_nat_t<8, -1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
+    case 17: return launch_big_nat_t<8, +1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
     default: return hipErrorInvalidValue;
     }
 }

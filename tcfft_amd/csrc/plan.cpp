@@ -827,11 +827,14 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
         src = (const uint32_t*)plan->workspace;
         return TCFFT_SUCCESS;
     };
-    // Contiguous-row first pass: the digit reversal is done in that pass's
-    // load (1D in place or not; 2D with its x reversal as a line permutation
-    // of the stores, from a staged copy when in place), so natural order
-    // costs no extra pass.
-    if (first.kind == 0 && !first.has_epi) {
+    // First pass over lines of up to 4096 points (k_rows) or of 8192..32768
+    // points (k_big<W <= 8>): the digit reversal is done in that pass's load
+    // (1D in place or not; 2D with its x reversal as a line permutation of the
+    // stores, from a staged copy when in place), so natural order costs no
+    // extra pass.
+    const bool fused_rows = first.kind == 0 && !first.has_epi;
+    const bool fused_big = first.kind == 2 && first.radix <= 8 && !first.has_epi;
+    if (fused_rows || fused_big) {
         const uint32_t* rsrc = (const uint32_t*)in;
         if (L0.dims == 2 && in == out) {
             int rc = stage(rsrc);
@@ -844,8 +847,12 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
             std::vector<int> rx(L0.rad_x.rbegin(), L0.rad_x.rend());
             x_pos = rev_spec(rx);
         }
-        hipError_t e = tcfft::launch_rows_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
-                                              plan->w4096, nx_log2, x_pos, st);
+        hipError_t e = fused_rows ? tcfft::launch_rows_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
+                                                           plan->w4096, nx_log2, x_pos, st)
+                                  : tcfft::launch_big_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
+                                                          plan->w4096, first.big_hi, first.big_lo,
+                                                          first.big_M_log2 / 2, nx_log2, x_pos,
+                                                          big_sched(plan, first.radix), st);
         if (e != hipSuccess) return hip_fail(e);
         return run_passes(plan, (uint32_t*)out, direction, st, 1);
     }

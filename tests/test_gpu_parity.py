@@ -166,7 +166,9 @@ def test_natural_order_equals_dropin(n, batch, inplace, direction):
                                                    ((512, 1024), 1, tcfft_amd.FORWARD),
                                                    ((1024, 1024), 2, tcfft_amd.INVERSE),
                                                    ((256, 4096), 1, tcfft_amd.FORWARD),
-                                                   ((4096, 256), 1, tcfft_amd.FORWARD)])
+                                                   ((4096, 256), 1, tcfft_amd.FORWARD),
+                                                   ((4, 8192), 2, tcfft_amd.FORWARD),
+                                                   ((8, 32768), 1, tcfft_amd.INVERSE)])
 def test_2d_natural_out_of_place(shape, batch, direction):
     """2D natural order out of place: the y reversal in the rows pass's load,
     the x reversal as the line order of its stores; equal to the drop-in."""
