@@ -96,9 +96,9 @@ int tcfft_exec(tcfft_plan plan, void *data, int direction, void *stream);
 /* Natural-order input and output; the result equals tcfft_exec on the input
  * permuted by the caller-side gen_rev (tcfft_doit_half.cpp:40-46), bit for
  * bit.  The digit reversal runs in the first pass's load for lines (1D, or
- * 2D rows) of up to 32768 points, else as one extra tiled reversal pass.  in == out is allowed; in and out must not otherwise
- * overlap.  The 2D in-place case and the 1D in-place case of 65536 points
- * and more stage the input in a plan-owned device workspace of the buffer's size
+ * 2D rows) of up to 65536 points, else as one extra tiled reversal pass.  in == out is allowed; in and out must not otherwise
+ * overlap.  The 2D in-place case and the 1D in-place case above 65536
+ * points stage the input in a plan-owned device workspace of the buffer's size
  * (allocated on first use), so concurrent in-place calls on one plan from
  * different streams race on it; all other cases allocate nothing and can be
  * captured into a hipGraph. */

@@ -670,7 +670,7 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big(u32* __restric
 }
 
 // ----------------------------------------------------------------------------
-// k_big_nat<W>: k_big<W> (W = 2, 4, 8) on natural-order input, in -> out (in
+// k_big_nat<W>: k_big<W> (W = 2, 4, 8, 16) on natural-order input, in -> out (in
 // place allowed: a workgroup reads its whole signal before its first store).
 // Each wave loads its natural 4096-element chunk with 16-byte loads and stores
 // it to LDS (the signal, W*16 KiB, fits at once); after a barrier wave w reads
@@ -700,6 +700,15 @@ __device__ __forceinline__ u32 nat_big_word(u32 m) {
     return m ^ (((m >> NatBigSwz<W>::A) & 3u) << 2) ^ (((m >> NatBigSwz<W>::B) & 3u) << 4);
 }
 
+// W = 16 (65536 points, 256 KiB per signal) goes through 128 KiB of LDS in
+// two rounds split by index bit 11: writer tile bit 3, reader lane bit
+// c16[3].  Round word: bit 11 removed, then bits 8-9 into 2-3 and 13-14 into
+// 4-5 (writes conflict-free, reads at most 2-way among the 32 active lanes).
+__device__ __forceinline__ u32 nat16_word(u32 n) {
+    const u32 m = (n & 2047u) | ((n >> 12) << 11);
+    return m ^ (((m >> 8) & 3u) << 2) ^ (((m >> 13) & 3u) << 4);
+}
+
 template <int W, int SIGN>
 __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
     const u32* __restrict__ in, u32* __restrict__ out, const float2* __restrict__ w4096,
@@ -713,24 +722,49 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
     const u32 w = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const u32 sig = blockIdx.x;
     BigTw bt;
-    big_tw_load<W, false>(bt, lane, (int)w, sig, w4096, wn_hi, wn_lo, wn_lo_bits, EpiParams{});
-    __builtin_amdgcn_sched_barrier(0);
-    {
+    u32 d[16][4];
+    if constexpr (W < 16) {
+        big_tw_load<W, false>(bt, lane, (int)w, sig, w4096, wn_hi, wn_lo, wn_lo_bits, EpiParams{});
+        __builtin_amdgcn_sched_barrier(0);
+        {
+            u32 v[16][4];
+            const u32* src = in + (size_t)sig * N + 4096u * w + 4u * lane;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) load4_nt(v[t], src + 256u * t);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < 16; ++t) store4(lds + nat_big_word<W>(4096u * w + 256u * t + 4u * lane), v[t]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                d[t][j] = lds[nat_big_word<W>(w + W * ((u32)t + 16u * c16 + 256u * (4u * g + j)))];
+    } else {
+        // data first, the twiddle tables after the exchange: the pending
+        // half of the loaded data and the reader registers fit the budget
         u32 v[16][4];
         const u32* src = in + (size_t)sig * N + 4096u * w + 4u * lane;
 #pragma unroll
         for (int t = 0; t < 16; ++t) load4_nt(v[t], src + 256u * t);
-        __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int t = 0; t < 16; ++t) store4(lds + nat_big_word<W>(4096u * w + 256u * t + 4u * lane), v[t]);
+        for (int r = 0; r < 2; ++r) {
+            if (r) __syncthreads();  // round 0's reads are done
+#pragma unroll
+            for (int tt = 0; tt < 8; ++tt)
+                store4(lds + nat16_word(4096u * w + 256u * (8u * r + tt) + 4u * lane), v[8 * r + tt]);
+            __syncthreads();
+            if ((u32)(c16 >> 3) == (u32)r) {
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+#pragma unroll
+                    for (int j = 0; j < 4; ++j)
+                        d[t][j] = lds[nat16_word(w + 16u * ((u32)t + 16u * c16 + 256u * (4u * g + j)))];
+            }
+        }
+        big_tw_load<W, false>(bt, lane_id(), (int)w, sig, w4096, wn_hi, wn_lo, wn_lo_bits, EpiParams{});
     }
-    __syncthreads();
-    u32 d[16][4];
-#pragma unroll
-    for (int t = 0; t < 16; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-            d[t][j] = lds[nat_big_word<W>(w + W * ((u32)t + 16u * c16 + 256u * (4u * g + j)))];
     __syncthreads();  // every wave has read its chunk before the exchange reuses the LDS
     u32 dst = sig;
     if (nx_log2) {
@@ -860,8 +894,10 @@ template <int W, int SIGN>
 static hipError_t launch_big_nat_t(const u32* in, u32* out, u32 n_signals, const float2* w4096, const float2* wn_hi,
                                    const float2* wn_lo, u32 lo_bits, u32 nx_log2, const RevSpec& x_pos,
                                    StartSched sched, hipStream_t st) {
-    const size_t lds = 4096u * W * sizeof(u32);  // the whole signal (>= the exchange's BigCfg<W>::LDS_WORDS)
-    static_assert(4096 * W >= BigCfg<W>::LDS_WORDS, "LDS covers the exchange");
+    // the whole signal (W < 16) or half of it (W = 16), >= the exchange's LDS
+    constexpr size_t words = W < 16 ? 4096u * W : 32768u;
+    static_assert(words >= (size_t)BigCfg<W>::LDS_WORDS, "LDS covers the exchange");
+    const size_t lds = words * sizeof(u32);
     auto k = k_big_nat<W, SIGN>;
     const KernelInfo ki = kernel_info(k, 64 * W, lds);
     if (ki.lds_status != hipSuccess) return ki.lds_status;
@@ -882,6 +918,8 @@ hipError_t launch_big_nat(int W, const u32* in, u32* out, u32 n_signals, int sig
     case 9: return launch_big_nat_t<4, +1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
     case 16: return launch_big_nat_t<8, -1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
     case 17: return launch_big_nat_t<8, +1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
+    case 32: return launch_big_nat_t<16, -1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
+    case 33: return launch_big_nat_t<16, +1>(in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, nx_log2, x_pos, sched, st);
     default: return hipErrorInvalidValue;
     }
 }

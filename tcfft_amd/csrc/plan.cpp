@@ -833,7 +833,7 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
     // stores, from a staged copy when in place), so natural order costs no
     // extra pass.
     const bool fused_rows = first.kind == 0 && !first.has_epi;
-    const bool fused_big = first.kind == 2 && first.radix <= 8 && !first.has_epi;
+    const bool fused_big = first.kind == 2 && !first.has_epi;
     if (fused_rows || fused_big) {
         const uint32_t* rsrc = (const uint32_t*)in;
         if (L0.dims == 2 && in == out) {
