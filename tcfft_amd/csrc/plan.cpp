@@ -799,7 +799,7 @@ int tcfft_exec_passes(tcfft_plan plan, void* data, int direction, int first, int
     if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
     if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
     const int np = (int)plan->lay.passes.size();
-    if (first < 0 || count < 0 || first + count > np) return TCFFT_INVALID_VALUE;
+    if (first < 0 || count < 0 || first > np || count > np - first) return TCFFT_INVALID_VALUE;
     DeviceGuard guard(plan->device);
     return run_passes(plan, (uint32_t*)data, direction, (hipStream_t)stream, (size_t)first, (size_t)count);
 }

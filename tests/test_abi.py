@@ -61,6 +61,29 @@ def test_create_rejects_bad_batch_and_pointers():
     assert L.tcfft_exec(ctypes.cast(junk, ctypes.c_void_p), ctypes.c_void_p(0x1000), -1, None) == 4
 
 
+def test_newer_entry_points_reject_bad_arguments_without_device():
+    """Radix-split creation, tuning, per-pass execution, schedules and the
+    uneven-batch setters validate their arguments before any device call."""
+    L = tcfft_amd.lib()
+    h = ctypes.c_void_p()
+    assert L.tcfft_plan_create_1d_split(ctypes.byref(h), 1 << 20, 1, 14) == 5   # no such first pass
+    assert L.tcfft_plan_create_1d_split(ctypes.byref(h), 100, 1, 16) == 1
+    assert L.tcfft_plan_create_1d_split(None, 1 << 20, 1, 16) == 3
+    assert L.tcfft_plan_create_1d_tuned(ctypes.byref(h), 3 << 12, 1, None, None) == 1
+    assert L.tcfft_plan_create_1d_tuned(ctypes.byref(h), 1 << 20, 0, None, None) == 2
+    assert h.value is None
+    assert L.tcfft_exec_passes(None, ctypes.c_void_p(0x1000), -1, 0, 1, None) == 4
+    assert L.tcfft_plan_autotune(None, ctypes.c_void_p(0x1000), None, None, None, None) == 4
+    assert L.tcfft_plan_set_option(None, 2, 1) == 4
+    assert L.tcfft_multi_set_start_delay(None, 1) == 4
+    assert L.tcfft_multi_get_schedule(None, None, None, None) == 4
+    rad = (ctypes.c_int * 32)()
+    nr, npass = ctypes.c_int(), ctypes.c_int()
+    assert L.tcfft_query_1d_split(1 << 20, 1, 13, rad, 32, ctypes.byref(nr), ctypes.byref(npass)) == 5
+    assert L.tcfft_query_1d_split(1 << 20, 1, 12, rad, 32, ctypes.byref(nr), ctypes.byref(npass)) == 0
+    assert npass.value == 2 and list(rad[: nr.value]) == [16] * 5
+
+
 @pytest.mark.parametrize("n", [256, 512, 1024, 2048, 4096, 8192, 1 << 16, 1 << 17, 1 << 20, 1 << 24, 1 << 27])
 def test_query_radix_lists_reproduce_the_dft(n):
     rad, npass = tcfft_amd.query_1d(n, 1)
