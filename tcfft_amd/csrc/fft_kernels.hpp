@@ -109,15 +109,10 @@ struct GatherTile {
 hipError_t launch_gather_tiled_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
                                   const RevSpec& spec, const GatherTile& gt, hipStream_t st);
 
-// Single-pass transform of N = 4096*W (W = 2, 4, 8) on natural-order input,
-// in -> out (in place allowed when nx_log2 = 0); 2D as launch_rows_nat.
+// Single-pass transform of N = 4096*W (W = 2, 4, 8, 16) on natural-order
+// input, in -> out (in place allowed when nx_log2 = 0); 2D as launch_rows_nat.
 hipError_t launch_big_nat(int W, const uint32_t* in, uint32_t* out, uint32_t n_signals, int sign, const float2* w4096,
                           const float2* wn_hi, const float2* wn_lo, uint32_t lo_bits, uint32_t nx_log2,
                           const RevSpec& x_pos, StartSched sched, hipStream_t st);
-
-hipError_t launch_gather_rev_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
-                                const RevSpec& spec, hipStream_t st);
-hipError_t launch_gather_rev_2d(const uint32_t* in, uint32_t* out, uint32_t nx_log2, uint32_t ny_log2,
-                                uint32_t total, const RevSpec& sx, const RevSpec& sy, hipStream_t st);
 
 }  // namespace tcfft

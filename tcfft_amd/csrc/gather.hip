@@ -27,27 +27,6 @@ __device__ __forceinline__ uint32_t rev_index(uint32_t p, const RevSpec& s) {
     return idx;
 }
 
-__global__ __launch_bounds__(256) void k_gather_1d(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                   uint32_t n_log2, uint32_t total, RevSpec spec) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const uint32_t mask = (1u << n_log2) - 1u;
-    const uint32_t p = i & mask;
-    out[i] = in[(i & ~mask) | rev_index(p, spec)];
-}
-
-__global__ __launch_bounds__(256) void k_gather_2d(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
-                                                   uint32_t nx_log2, uint32_t ny_log2, uint32_t total, RevSpec sx,
-                                                   RevSpec sy) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const uint32_t y = i & ((1u << ny_log2) - 1u);
-    const uint32_t x = (i >> ny_log2) & ((1u << nx_log2) - 1u);
-    const uint32_t img = i >> (nx_log2 + ny_log2);
-    const uint32_t src = (img << (nx_log2 + ny_log2)) | (rev_index(x, sx) << ny_log2) | rev_index(y, sy);
-    out[i] = in[src];
-}
-
 // Tiled 1D digit reversal through LDS.  Position p's bit permutation: bit i
 // of rev(p) is bit pi(i) of p.  A tile of 2^(a+b) elements fixes every bit
 // of p except the low a bits (consecutive positions: the writes) and the b
@@ -122,19 +101,6 @@ hipError_t launch_gather_tiled_1d(const uint32_t* in, uint32_t* out, uint32_t n_
         return hipErrorInvalidValue;
     const uint32_t blocks = total >> (gt.a + gt.b);
     hipLaunchKernelGGL(k_gather_tiled, dim3(blocks), dim3(256), 0, st, in, out, n_log2, spec, gt);
-    return hipGetLastError();
-}
-
-hipError_t launch_gather_rev_1d(const uint32_t* in, uint32_t* out, uint32_t n_log2, uint32_t total,
-                                const RevSpec& spec, hipStream_t st) {
-    hipLaunchKernelGGL(k_gather_1d, dim3((total + 255) / 256), dim3(256), 0, st, in, out, n_log2, total, spec);
-    return hipGetLastError();
-}
-
-hipError_t launch_gather_rev_2d(const uint32_t* in, uint32_t* out, uint32_t nx_log2, uint32_t ny_log2,
-                                uint32_t total, const RevSpec& sx, const RevSpec& sy, hipStream_t st) {
-    hipLaunchKernelGGL(k_gather_2d, dim3((total + 255) / 256), dim3(256), 0, st, in, out, nx_log2, ny_log2, total,
-                       sx, sy);
     return hipGetLastError();
 }
 

@@ -856,24 +856,20 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
         if (e != hipSuccess) return hip_fail(e);
         return run_passes(plan, (uint32_t*)out, direction, st, 1);
     }
+    // 1D lines above 65536 points: the LDS-tiled reversal pass, then the
+    // plan's passes (every plan takes one of these two routes: 2D rows are
+    // at most 65536 points, and a longer line's radix list always admits a
+    // reversal tile).
+    const Layout& L = plan->lay;
+    tcfft::GatherTile gt{};
+    if (L.dims != 1 || !gather_tile(L.rad_y, gt)) return TCFFT_INVALID_VALUE;
     const uint32_t* src = (const uint32_t*)in;
     if (in == out) {
         int rc = stage(src);
         if (rc) return rc;
     }
-    const Layout& L = plan->lay;
-    hipError_t e;
-    tcfft::GatherTile gt{};
-    if (L.dims == 1 && gather_tile(L.rad_y, gt))
-        e = tcfft::launch_gather_tiled_1d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.n), (uint32_t)plan->total,
-                                          rev_spec(L.rad_y), gt, st);
-    else if (L.dims == 1)
-        e = tcfft::launch_gather_rev_1d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.n), (uint32_t)plan->total,
-                                        rev_spec(L.rad_y), st);
-    else
-        e = tcfft::launch_gather_rev_2d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.nx),
-                                        (uint32_t)ilog2((uint32_t)L.ny), (uint32_t)plan->total, rev_spec(L.rad_x),
-                                        rev_spec(L.rad_y), st);
+    hipError_t e = tcfft::launch_gather_tiled_1d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.n),
+                                                 (uint32_t)plan->total, rev_spec(L.rad_y), gt, st);
     if (e != hipSuccess) return hip_fail(e);
     return run_passes(plan, (uint32_t*)out, direction, st);
 }
