@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--batch", type=int, default=BATCH)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-configs", action="store_true", help="headline workload only")
-    ap.add_argument("--workload", default="plain", choices=["plain", "split", "uneven", "conv"],
+    ap.add_argument("--workload", default="plain", choices=["plain", "split", "uneven", "conv", "r2c"],
                     help="plain: the headline fp16 exec (+ every config); split / uneven / conv: that leg alone")
     ap.add_argument("--split-chunk", type=int, default=None, help="signals per split pass pair")
     ap.add_argument("--calib", action="store_true",
@@ -464,6 +464,52 @@ def bench_conv(torch, reps):
             "note": "forward = 2 FFTs + pointwise product; forward+backward = 6 FFTs + products (torch autograd)"}
 
 
+def bench_r2c(torch, tcfft_amd, reps):
+    """Real-input transforms on the FFT-conv input shape as real signals:
+    16384 = 64 x 256 signals of n = 8192 fp16 reals, forward (bins 0..n/2)
+    and inverse, beside the complex transform of the same data fed as complex
+    (exec_natural, out of place), which is what a caller without R2C runs."""
+    n, batch = 8192, 64 * 256
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, (batch, n)).astype(np.float16)
+    plan = tcfft_amd.PlanR2C(n, batch)
+    bins = n // 2 + 1
+    real = rotation(torch, (batch, n), torch.float16, torch.from_numpy(x))
+    spec = [torch.empty(batch, bins, 2, dtype=torch.float16, device="cuda") for _ in range(3)]
+    plan.exec_r2c(real[0].data_ptr(), spec[0].data_ptr())
+    torch.cuda.synchronize()
+    k = 4
+    err_f = rel(to_c(spec[0][:k]), np.fft.rfft(x[:k].astype(np.float64), axis=-1))
+    for sp in spec[1:]:
+        sp.copy_(spec[0])
+    back = torch.empty(batch, n, dtype=torch.float16, device="cuda")
+    plan.exec_c2r(spec[0].data_ptr(), back.data_ptr())
+    torch.cuda.synchronize()
+    ref_b = n * np.fft.irfft(to_c(spec[0][:k]), n, axis=-1)
+    err_i = rel(back[:k].double().cpu().numpy(), ref_b)
+    ms_f = graph_time(torch, lambda i, st: plan.exec_r2c(real[i % len(real)].data_ptr(), spec[i % 3].data_ptr(), st),
+                      reps, 3)
+    ms_i = graph_time(torch, lambda i, st: plan.exec_c2r(spec[i % 3].data_ptr(), real[i % len(real)].data_ptr(), st),
+                      reps, 3)
+    plan.destroy()
+    del real, spec, back
+    alg = (2.0 * n + 4.0 * bins) * batch
+    cp = tcfft_amd.Plan1D(n, batch)
+    xc = np.zeros((batch, n, 2), dtype=np.float16)
+    xc[..., 0] = x
+    src = rotation(torch, (batch, n, 2), torch.float16, torch.from_numpy(xc))
+    dst = torch.empty_like(src[0])
+    ms_c = graph_time(torch, lambda i, st: cp.exec_natural(src[i % len(src)].data_ptr(), dst.data_ptr(),
+                                                           tcfft_amd.FORWARD, st), reps, 3)
+    cp.destroy()
+    del src, dst
+    fwd = summary(ms_f, n, batch, alg)
+    return {"n": n, "signals": batch, "ms_r2c": round(ms_f, 5), "ms_c2r": round(ms_i, 5),
+            "hbm_frac_r2c": fwd["hbm_frac"], "hbm_frac_c2r": summary(ms_i, n, batch, alg)["hbm_frac"],
+            "alg_bytes": int(alg), "rel_l2_r2c": err_f, "rel_l2_c2r": err_i,
+            "ms_complex_of_real_data": round(ms_c, 5), "speedup_vs_complex": round(ms_c / ms_f, 3)}
+
+
 def hipfft_fp16(torch, dims, batch, reps):
     """The vendor library on the same workload: hipFFT (rocFFT) half-precision
     C2C through hipfftXtMakePlanMany(HIP_C_16F), in place, natural order,
@@ -547,7 +593,8 @@ def main():
     if args.workload != "plain":
         leg = {"split": lambda: bench_split(torch, tcfft_amd, args.steps, args.split_chunk),
                "uneven": lambda: bench_uneven(torch, tcfft_amd, args.steps),
-               "conv": lambda: bench_conv(torch, max(3, args.steps // 20))}[args.workload]()
+               "conv": lambda: bench_conv(torch, max(3, args.steps // 20)),
+               "r2c": lambda: bench_r2c(torch, tcfft_amd, args.steps)}[args.workload]()
         print(json.dumps({"workload": args.workload, **leg}), flush=True)
         return
 
@@ -631,6 +678,7 @@ def main():
         cfg["c3_split_n2^16_b256"] = bench_split(torch, tcfft_amd, reps, ms_plain=ms)
         cfg["c4_uneven_4096_signals"] = bench_uneven(torch, tcfft_amd, reps)
         cfg["c5_fftconv_64x256x8192"] = bench_conv(torch, 5)
+        cfg["r2c_n8192_16384_signals"] = bench_r2c(torch, tcfft_amd, reps)
         cfg["hipfft_fp16_n2^16_b256"] = hipfft_fp16(torch, [n], batch, reps)
         cfg["hipfft_fp16_n4096_b1024"] = hipfft_fp16(torch, [4096], 1024, reps)
         cfg["hipfft_fp16_2d_1024x1024_b8"] = hipfft_fp16(torch, [1024, 1024], 8, reps)

@@ -111,6 +111,26 @@ int tcfft_exec_natural(tcfft_plan plan, const void *in, void *out, int direction
  * plans only (else TCFFT_INVALID_SIZE). */
 int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
 
+/* Real-input transforms (no counterpart in the reference, whose callers feed
+ * real data as complex with zero imaginary parts, e.g. the generator of
+ * doit.h:7-19 fills both).  A plan for n_batch real signals of n = 512, 1024,
+ * 2048, 4096 or 8192 points (else TCFFT_INVALID_SIZE); it is also the complex
+ * plan of n/2 points for every other entry point.  Each direction is one HBM
+ * round trip of the half-size data:
+ *   tcfft_exec_r2c: in = n_batch * n FP16 reals (16-byte aligned), out =
+ *     bins 0..n/2 of each signal's DFT (sign e^{-2 pi i nk/N}), n/2 + 1 FP16
+ *     complex per signal, signal b at out + 4*(n/2+1)*b bytes (4-byte aligned);
+ *   tcfft_exec_c2r: in = n/2 + 1 bins per signal as above, taken as the
+ *     lower half of a Hermitian spectrum (imaginary parts of bins 0 and n/2
+ *     ignored), out = y[t] = sum_{k<n} X[k] e^{+2 pi i kt/n} (unnormalised,
+ *     n times numpy's irfft), n FP16 reals per signal.
+ * Out of place only (the buffers must not overlap: TCFFT_INVALID_POINTER);
+ * no allocation, graph-capturable.  A plan not made by
+ * tcfft_plan_create_r2c_1d gives TCFFT_INVALID_PLAN. */
+int tcfft_plan_create_r2c_1d(tcfft_plan *plan, int n, int n_batch);
+int tcfft_exec_r2c(tcfft_plan plan, const void *in, void *out, void *stream);
+int tcfft_exec_c2r(tcfft_plan plan, const void *in, void *out, void *stream);
+
 /* Natural-order input, spectrum output in the plan's digit-reversed order
  * (position p holds X[rev[p]], rev = tcfft_gen_rev of the plan's radices):
  * the order tcfft_exec consumes, so a transform, a pointwise product in that

@@ -46,6 +46,8 @@ def lib():
         L.oracle_fft.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_fft_2d.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_dft_ld.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int]
+        L.oracle_rfft.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int]
+        L.oracle_irfft.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int]
         L.oracle_rel_l2.argtypes = [_f64p, _f64p, ctypes.c_size_t]
         L.oracle_rel_l2.restype = ctypes.c_double
         L.oracle_default_radices.argtypes = [ctypes.c_int, _i32p, ctypes.c_int]
@@ -148,6 +150,26 @@ def fft_2d(x, sign: int = -1) -> np.ndarray:
     b = a.size // (nx * ny)
     out = np.empty_like(a)
     lib().oracle_fft_2d(_dp(a), _dp(out), nx, ny, b, sign)
+    return out
+
+
+def rfft(x) -> np.ndarray:
+    """Bins 0..n/2 of the DFT of real signals along the last axis (the
+    real-as-complex transform the reference's callers run, truncated)."""
+    a = np.ascontiguousarray(x, dtype=np.float64)
+    n = a.shape[-1]
+    b = a.size // n
+    out = np.empty(a.shape[:-1] + (n // 2 + 1,), dtype=np.complex128)
+    lib().oracle_rfft(_dp(a), _dp(out), n, b)
+    return out
+
+
+def irfft(X, n: int) -> np.ndarray:
+    """Unnormalised inverse of rfft: n * numpy.fft.irfft(X, n)."""
+    a = _c128(X)
+    b = a.size // (n // 2 + 1)
+    out = np.empty(a.shape[:-1] + (n,), dtype=np.float64)
+    lib().oracle_irfft(_dp(a), _dp(out), n, b)
     return out
 
 

@@ -303,6 +303,42 @@ void oracle_fft(const double *in, double *out, int n, int n_batch, int sign)
     free(rev);
 }
 
+void oracle_rfft(const double *in, double *out, int n, int n_batch)
+{
+    double *full = (double *)malloc(sizeof(double) * 2 * (size_t)n * n_batch);
+    for (size_t i = 0; i < (size_t)n * n_batch; ++i) {
+        full[2 * i] = in[i];
+        full[2 * i + 1] = 0.0;
+    }
+    oracle_fft(full, full, n, n_batch, -1);
+    const int bins = n / 2 + 1;
+    for (int b = 0; b < n_batch; ++b)
+        memcpy(out + 2 * (size_t)b * bins, full + 2 * (size_t)b * n, sizeof(double) * 2 * (size_t)bins);
+    free(full);
+}
+
+void oracle_irfft(const double *in, double *out, int n, int n_batch)
+{
+    const int h = n / 2, bins = h + 1;
+    double *full = (double *)malloc(sizeof(double) * 2 * (size_t)n * n_batch);
+    for (int b = 0; b < n_batch; ++b) {
+        const double *x = in + 2 * (size_t)b * bins;
+        double *f = full + 2 * (size_t)b * n;
+        for (int k = 0; k <= h; ++k) {
+            f[2 * k] = x[2 * k];
+            f[2 * k + 1] = (k == 0 || k == h) ? 0.0 : x[2 * k + 1];
+        }
+        for (int k = 1; k < h; ++k) {
+            f[2 * (n - k)] = x[2 * k];
+            f[2 * (n - k) + 1] = -x[2 * k + 1];
+        }
+    }
+    oracle_fft(full, full, n, n_batch, +1);
+    for (size_t i = 0; i < (size_t)n * n_batch; ++i)
+        out[i] = full[2 * i];
+    free(full);
+}
+
 void oracle_fft_2d(const double *in, double *out, int nx, int ny, int n_batch, int sign)
 {
     int rx[32], ry[32];

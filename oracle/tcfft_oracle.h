@@ -25,6 +25,9 @@
  *   oracle_dft_ld          textbook closed forms (impulse, constant, tone) and
  *                          the numpy golden vectors
  *   oracle_rel_l2          hand-computed cases
+ *   oracle_rfft / _irfft   numpy rfft / irfft, Hermitian symmetry of the full
+ *                          transform, real-cosine and Nyquist closed forms,
+ *                          irfft(rfft(x)) = n x
  */
 #ifndef TCFFT_ORACLE_H
 #define TCFFT_ORACLE_H
@@ -80,6 +83,18 @@ void oracle_dit_2d(double *data, int nx, int ny, int n_batch,
  * radix-16-first list.  n must be a power of two.  in and out may alias. */
 void oracle_fft(const double *in, double *out, int n, int n_batch, int sign);
 void oracle_fft_2d(const double *in, double *out, int nx, int ny, int n_batch, int sign);
+
+/* Real-input transform, bins 0..n/2 of each signal's DFT (n/2 + 1 complex
+ * per signal; in: n_batch * n doubles).  No counterpart in the reference,
+ * whose callers feed real data as complex with zero imaginary parts (the
+ * FFT-conv workload; doit.h:7-19 fills both components): this is exactly
+ * that, oracle_fft of the real-as-complex input, truncated to n/2 + 1 bins. */
+void oracle_rfft(const double *in, double *out, int n, int n_batch);
+
+/* Inverse of the above, unnormalised: y[t] = sum_{k<n} X[k] e^{+2 pi i kt/n}
+ * over the Hermitian extension X[n-k] = conj X[k] of bins 0..n/2, with the
+ * imaginary parts of bins 0 and n/2 dropped (n * numpy's irfft). */
+void oracle_irfft(const double *in, double *out, int n, int n_batch);
 
 /* O(n^2) DFT in long double with exactly reduced angles (pin for the above). */
 void oracle_dft_ld(const double *in, double *out, int n, int sign);

@@ -42,6 +42,7 @@ EXPORTS = (
     "tcfft_multi_set_schedule", "tcfft_multi_autotune", "tcfft_multi_set_start_delay", "tcfft_multi_get_schedule",
     "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad", "tcfft_plan_autotune",
     "tcfft_plan_create_1d_split", "tcfft_plan_create_1d_tuned", "tcfft_query_1d_split",
+    "tcfft_plan_create_r2c_1d", "tcfft_exec_r2c", "tcfft_exec_c2r",
 )
 OPT_SPLIT_CHUNK = 1
 OPT_BIG_DELAY = 2
@@ -93,6 +94,9 @@ def lib():
         L.tcfft_plan_create_1d_tuned.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, _i32p,
                                                  ctypes.POINTER(ctypes.c_float)]
         L.tcfft_query_1d_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p]
+        L.tcfft_plan_create_r2c_1d.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]
+        L.tcfft_exec_r2c.argtypes = [_vp, _vp, _vp, _vp]
+        L.tcfft_exec_c2r.argtypes = [_vp, _vp, _vp, _vp]
         L.tcfft_exec_dif.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_conv_exec.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
         L.tcfft_conv_filter_grad.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, _vp]
@@ -272,6 +276,34 @@ class Plan1D(_Plan):
     @property
     def numel(self) -> int:
         return self.n * self.batch
+
+
+class PlanR2C(_Plan):
+    """Real-input transforms of batch signals of n = 512..8192 real points
+    (no reference counterpart).  exec_r2c: n fp16 reals -> bins 0..n/2
+    (n/2 + 1 fp16 complex per signal); exec_c2r: the unnormalised inverse
+    (n times numpy's irfft).  Out of place.  As a complex plan (exec,
+    exec_natural, ...) it is the plan of n/2 points."""
+
+    dims = 1
+
+    def __init__(self, n: int, batch: int = 1):
+        super().__init__()
+        self.n, self.batch = n, batch
+        _check(lib().tcfft_plan_create_r2c_1d(ctypes.byref(self._h), n, batch),
+               f"tcfft_plan_create_r2c_1d(n={n}, batch={batch})")
+        self.radices = self._radices(0)
+        self.n_radices = len(self.radices)
+
+    @property
+    def bins(self) -> int:
+        return self.n // 2 + 1
+
+    def exec_r2c(self, in_ptr: int, out_ptr: int, stream: int = 0):
+        _check(lib().tcfft_exec_r2c(self._h, _vp(in_ptr), _vp(out_ptr), _vp(stream)), "tcfft_exec_r2c")
+
+    def exec_c2r(self, in_ptr: int, out_ptr: int, stream: int = 0):
+        _check(lib().tcfft_exec_c2r(self._h, _vp(in_ptr), _vp(out_ptr), _vp(stream)), "tcfft_exec_c2r")
 
 
 class Plan2D(_Plan):

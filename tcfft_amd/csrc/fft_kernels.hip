@@ -219,6 +219,38 @@ __device__ __forceinline__ u32 rev_pos_pow2(u32 x, const RevSpec& s) {
     return idx;
 }
 
+// Reads a wave's natural-order chunk (ntl tiles, whole lines of 256*T) from
+// its LDS slice into the digit-reversed register layout of k_rows.
+template <int T>
+__device__ __forceinline__ void nat_rows_read(u32 (&d)[16][4], const u32* wl, u32 ntl, int lane) {
+    constexpr u32 L = 256u * T;
+    const int c16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const u32 m = (u32)(t / T) * L + (u32)(t % T) + T * (u32)c16 + 16u * T * (4u * g + j);
+            d[t][j] = (u32)t < ntl ? wl[nat_lds_word(m)] : 0u;
+        }
+}
+
+// 16-byte loads of ntl tiles at src (this wave's chunk) into the LDS slice,
+// then nat_rows_read.
+template <int T>
+__device__ __forceinline__ void nat_rows_load(u32 (&d)[16][4], const u32* __restrict__ src, u32* wl, u32 ntl,
+                                              int lane) {
+    src += 4u * lane;
+    u32 v[16][4];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+        if ((u32)t < ntl) load4_nt(v[t], src + 256u * t);
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+        if ((u32)t < ntl) store4(wl + nat_lds_word(256u * t + 4u * lane), v[t]);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    nat_rows_read<T>(d, wl, ntl, lane);
+}
+
 template <int T, int SIGN>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* __restrict__ in, u32* __restrict__ out,
                                                                   u32 n_tiles, const float2* __restrict__ w4096,
@@ -233,24 +265,7 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* _
     const u32 ntl = min(16u, n_tiles - tile0);
     u32* const wl = lds + wib * 4096u;
     const int c16 = lane & 15, g = lane >> 4;
-    auto load = [&](u32 (&d)[16][4]) {
-        const u32* src = in + (size_t)tile0 * 256u + 4u * lane;
-        u32 v[16][4];
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if ((u32)t < ntl) load4_nt(v[t], src + 256u * t);
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-            if ((u32)t < ntl) store4(wl + nat_lds_word(256u * t + 4u * lane), v[t]);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-#pragma unroll
-        for (int t = 0; t < 16; ++t)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const u32 m = (u32)(t / T) * L + (u32)(t % T) + T * (u32)c16 + 16u * T * (4u * g + j);
-                d[t][j] = (u32)t < ntl ? wl[nat_lds_word(m)] : 0u;
-            }
-    };
+    auto load = [&](u32 (&d)[16][4]) { nat_rows_load<T>(d, in + (size_t)tile0 * 256u, wl, ntl, lane); };
     const u32 lane_off = 16u * c16 + 4u * g;
     auto tile_ptr = [&](int t) -> u32* {
         const u32 line = (tile0 + (u32)t) / T;
@@ -262,6 +277,120 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* _
         return out + (size_t)dst * L + 256u * ((u32)t % T) + lane_off;
     };
     rows_body<T, false, SIGN>(tile_ptr, ntl, tile0, w4096, EpiParams{}, load);
+}
+
+// ----------------------------------------------------------------------------
+// Real-input transforms of N = 512*T points (T = 1..16) on the row kernel.  A
+// real signal x[0..N) is read as L = N/2 complex points z[m] = x[2m] + i x[2m+1]
+// (the same bytes, 16-byte aligned).  Bins k = 0..L of X = DFT_N(x) follow
+// from Z = DFT_L(z) through the even/odd split X = E + W_N^k O:
+//   X[k] = ((Z[k] + conj Z[L-k]) - i W_N^k (Z[k] - conj Z[L-k])) / 2,  Z[L] = Z[0].
+// The inverse takes bins 0..L of a Hermitian spectrum (the imaginary parts of
+// bins 0 and L are ignored) and forms
+//   Z'[k] = (X[k] + conj X[L-k]) + i W_N^-k (X[k] - conj X[L-k]),  k < L,
+// whose unnormalised inverse DFT_L has re/im = y[2m], y[2m+1] of
+// y[t] = sum_{k<N} X[k] e^{2 pi i k t/N} (N times numpy's irfft).
+// Complex side: L+1 FP16 complex words per signal, signal s at word s*(L+1).
+// W_N^k for k = 64j + lane is wn[lane] * wn[64]^j (wn = W_N^0..64, FP64-made
+// FP32 table; the recurrence runs at most 64 steps).
+
+// k_r2c<T>: the k_rows_nat pass with its output kept in the wave's LDS slice
+// (natural order, same swizzle), then the untangle in FP32, 64 consecutive
+// bins per store instruction.
+template <int T>
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_r2c(const u32* __restrict__ in, u32* __restrict__ out,
+                                                             u32 n_tiles, const float2* __restrict__ w4096,
+                                                             const float2* __restrict__ wn) {
+    extern __shared__ u32 lds[];
+    constexpr u32 L = 256u * T;
+    const int lane = threadIdx.x & 63;
+    const u32 wib = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 wave = blockIdx.x * (blockDim.x >> 6) + wib;
+    const u32 tile0 = wave * 16u;
+    if (tile0 >= n_tiles) return;
+    const u32 ntl = min(16u, n_tiles - tile0);
+    u32* const wl = lds + wib * 4096u;
+    const int c16 = lane & 15, g = lane >> 4;
+    const float2 wb = wn[lane], ws = wn[64];
+    auto load = [&](u32 (&d)[16][4]) { nat_rows_load<T>(d, in + (size_t)tile0 * 256u, wl, ntl, lane); };
+    const u32 lane_off = 16u * c16 + 4u * g;
+    auto tile_ptr = [&](int t) -> u32* { return wl + nat_lds_word(256u * (u32)t + lane_off); };
+    rows_body<T, false, -1>(tile_ptr, ntl, tile0, w4096, EpiParams{}, load);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const cf step = {ws.x, ws.y};
+    const u32 line0 = tile0 / T;
+#pragma unroll
+    for (u32 i = 0; i < 16u / T; ++i) {
+        if (i * T >= ntl) break;
+        u32* const dst = out + (size_t)(line0 + i) * (L + 1u);
+        cf w = {wb.x, wb.y};
+        cf z0 = {};
+#pragma unroll
+        for (u32 j = 0; j < L / 64u; ++j) {
+            const u32 k = 64u * j + (u32)lane;
+            const cf a = unpack(wl[nat_lds_word(i * L + k)]);
+            const cf b = unpack(wl[nat_lds_word(i * L + ((L - k) & (L - 1u)))]);
+            if (j == 0) z0 = a;
+            const cf sum = {a.x + b.x, a.y - b.y};   // Z[k] + conj Z[L-k]
+            const cf dif = {a.x - b.x, a.y + b.y};   // Z[k] - conj Z[L-k]
+            const cf p = cmul(w, dif);
+            __builtin_nontemporal_store(pack(0.5f * (sum.x + p.y), 0.5f * (sum.y - p.x)), dst + k);
+            w = cmul(w, step);
+        }
+        if (lane == 0) __builtin_nontemporal_store(pack(z0.x - z0.y, 0.f), dst + L);  // X[L] = Re Z[0] - Im Z[0]
+    }
+}
+
+// k_c2r<T>: Z' formed from bin pairs read straight from global memory (the
+// mirrored read of a line hits lines the wave has just fetched), written to
+// the LDS slice in natural order, then the k_rows_nat inverse pass to the
+// real output.
+template <int T>
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_c2r(const u32* __restrict__ in, u32* __restrict__ out,
+                                                             u32 n_tiles, const float2* __restrict__ w4096,
+                                                             const float2* __restrict__ wn) {
+    extern __shared__ u32 lds[];
+    constexpr u32 L = 256u * T;
+    const int lane = threadIdx.x & 63;
+    const u32 wib = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 wave = blockIdx.x * (blockDim.x >> 6) + wib;
+    const u32 tile0 = wave * 16u;
+    if (tile0 >= n_tiles) return;
+    const u32 ntl = min(16u, n_tiles - tile0);
+    u32* const wl = lds + wib * 4096u;
+    const int c16 = lane & 15, g = lane >> 4;
+    const float2 wb = wn[lane], ws = wn[64];
+    const u32 line0 = tile0 / T;
+    auto load = [&](u32 (&d)[16][4]) {
+        const cf step = {ws.x, -ws.y};
+#pragma unroll
+        for (u32 i = 0; i < 16u / T; ++i) {
+            if (i * T >= ntl) break;
+            const u32* const src = in + (size_t)(line0 + i) * (L + 1u);
+            u32 va[L / 64u], vb[L / 64u];
+#pragma unroll
+            for (u32 j = 0; j < L / 64u; ++j) {
+                va[j] = __builtin_nontemporal_load(src + 64u * j + (u32)lane);
+                vb[j] = src[L - 64u * j - (u32)lane];
+            }
+            cf w = {wb.x, -wb.y};
+#pragma unroll
+            for (u32 j = 0; j < L / 64u; ++j) {
+                cf a = unpack(va[j]), b = unpack(vb[j]);
+                if (j == 0 && lane == 0) a.y = b.y = 0.f;  // bins 0 and L: real parts only
+                const cf sum = {a.x + b.x, a.y - b.y};   // X[k] + conj X[L-k]
+                const cf dif = {a.x - b.x, a.y + b.y};   // X[k] - conj X[L-k]
+                const cf p = cmul(w, dif);
+                wl[nat_lds_word(i * L + 64u * j + (u32)lane)] = pack(sum.x - p.y, sum.y + p.x);
+                w = cmul(w, step);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        nat_rows_read<T>(d, wl, ntl, lane);
+    };
+    const u32 lane_off = 16u * c16 + 4u * g;
+    auto tile_ptr = [&](int t) -> u32* { return out + (size_t)(tile0 + (u32)t) * 256u + lane_off; };
+    rows_body<T, false, +1>(tile_ptr, ntl, tile0, w4096, EpiParams{}, load);
 }
 
 // ----------------------------------------------------------------------------
@@ -886,6 +1015,30 @@ hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign
     case 17: return launch_rows_nat_t<8, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
     case 32: return launch_rows_nat_t<16, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
     case 33: return launch_rows_nat_t<16, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <int T>
+static hipError_t launch_r2c_t(int sign, const u32* in, u32* out, u32 n_tiles, const float2* w4096,
+                               const float2* wn, hipStream_t st) {
+    constexpr size_t lds = 4 * 4096 * sizeof(u32);  // 16 KiB per wave
+    auto k = sign < 0 ? k_r2c<T> : k_c2r<T>;
+    const KernelInfo ki = kernel_info(k, 256, lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    const u32 waves = (n_tiles + 15) / 16;
+    hipLaunchKernelGGL(k, dim3((waves + 3) / 4), dim3(256), lds, st, in, out, n_tiles, w4096, wn);
+    return hipGetLastError();
+}
+
+hipError_t launch_r2c(int T, int sign, const u32* in, u32* out, u32 n_tiles, const float2* w4096, const float2* wn,
+                      hipStream_t st) {
+    switch (T) {
+    case 1: return launch_r2c_t<1>(sign, in, out, n_tiles, w4096, wn, st);
+    case 2: return launch_r2c_t<2>(sign, in, out, n_tiles, w4096, wn, st);
+    case 4: return launch_r2c_t<4>(sign, in, out, n_tiles, w4096, wn, st);
+    case 8: return launch_r2c_t<8>(sign, in, out, n_tiles, w4096, wn, st);
+    case 16: return launch_r2c_t<16>(sign, in, out, n_tiles, w4096, wn, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -97,6 +97,12 @@ struct RevSpec {
 hipError_t launch_rows_nat(int T, const uint32_t* in, uint32_t* out, uint32_t n_tiles, int sign, const float2* w4096,
                            uint32_t nx_log2, const RevSpec& x_pos, hipStream_t st);
 
+// Real-input transforms of N = 512*T points (T = 1, 2, 4, 8, 16), n_tiles =
+// batch * T.  sign -1: real in (batch * N/2 words, 16-byte aligned) -> bins
+// 0..N/2 out (batch * (N/2 + 1) words); sign +1: the inverse.  wn = W_N^0..64.
+hipError_t launch_r2c(int T, int sign, const uint32_t* in, uint32_t* out, uint32_t n_tiles, const float2* w4096,
+                      const float2* wn, hipStream_t st);
+
 // Tile of the LDS-tiled 1D digit reversal: a low bits of the position and
 // the b position bits in_bit[0..b-1] that form the low bits of the input
 // index (disjoint, a + b <= 12); free_mask = the remaining position bits.

@@ -205,6 +205,8 @@ struct tcfft_plan_s {
     int big_delay = -1;   // start schedule of the first-pass kernels: -1 = default
     int big_groups = 3;
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
+    int real_n = 0;       // real-input plan: the real length (its complex layout has n = real_n / 2)
+    float2* w_real = nullptr;  // W_{real_n}^0..64
 };
 
 namespace {
@@ -572,6 +574,53 @@ int tcfft_exec(tcfft_plan plan, void* data, int direction, void* stream) {
     if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
     DeviceGuard guard(plan->device);
     return run_passes(plan, (uint32_t*)data, direction, (hipStream_t)stream);
+}
+
+int tcfft_plan_create_r2c_1d(tcfft_plan* plan, int n, int n_batch) {
+    if (!plan) return TCFFT_INVALID_POINTER;
+    *plan = nullptr;
+    if (!is_pow2(n) || n < 512 || n > 8192) return TCFFT_INVALID_SIZE;
+    Layout lay;
+    int rc = build_layout_1d(n / 2, n_batch, lay);
+    if (rc) return rc;
+    rc = create_common(plan, lay);
+    if (rc) return rc;
+    tcfft_plan_s* p = *plan;
+    p->real_n = n;
+    rc = upload(p, host_table((uint32_t)ilog2((uint32_t)n), 1, 65), &p->w_real);
+    if (rc) {
+        free_plan(p);
+        *plan = nullptr;
+    }
+    return rc;
+}
+
+int tcfft_exec_r2c(tcfft_plan plan, const void* in, void* out, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!plan->real_n) return TCFFT_INVALID_PLAN;
+    const size_t half = (size_t)plan->real_n / 2, batch = (size_t)plan->lay.batch;
+    const uintptr_t a = (uintptr_t)in, b = (uintptr_t)out;
+    if (!in || !out || (a & 15u) || (b & 3u)) return TCFFT_INVALID_POINTER;
+    if (a < b + (half + 1) * batch * 4u && b < a + half * batch * 4u) return TCFFT_INVALID_POINTER;  // overlap
+    DeviceGuard guard(plan->device);
+    hipError_t e = tcfft::launch_r2c((int)(half / 256), -1, (const uint32_t*)in, (uint32_t*)out,
+                                     (uint32_t)(batch * (half / 256)), plan->w4096, plan->w_real, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_exec_c2r(tcfft_plan plan, const void* in, void* out, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (!plan->real_n) return TCFFT_INVALID_PLAN;
+    const size_t half = (size_t)plan->real_n / 2, batch = (size_t)plan->lay.batch;
+    const uintptr_t a = (uintptr_t)in, b = (uintptr_t)out;
+    if (!in || !out || (a & 3u) || (b & 15u)) return TCFFT_INVALID_POINTER;
+    if (a < b + half * batch * 4u && b < a + (half + 1) * batch * 4u) return TCFFT_INVALID_POINTER;  // overlap
+    DeviceGuard guard(plan->device);
+    hipError_t e = tcfft::launch_r2c((int)(half / 256), +1, (const uint32_t*)in, (uint32_t*)out,
+                                     (uint32_t)(batch * (half / 256)), plan->w4096, plan->w_real, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
 }
 
 int tcfft_exec_split(tcfft_plan plan, const void* in, void* out, int direction, void* stream) {

@@ -153,3 +153,15 @@ def test_gen_rev_rejects_bad_lists():
     out = (ctypes.c_int * 256)()
     assert L.tcfft_gen_rev(512, r, 2, out) == 1
     assert L.tcfft_gen_rev(256, None, 2, out) == 3
+
+
+def test_real_input_entry_points_validate_without_device():
+    L = tcfft_amd.lib()
+    h = ctypes.c_void_p()
+    for bad in (0, 256, 1000, 16384, 1 << 20):
+        assert L.tcfft_plan_create_r2c_1d(ctypes.byref(h), bad, 1) == 1
+    assert L.tcfft_plan_create_r2c_1d(ctypes.byref(h), 1024, 0) == 2
+    assert L.tcfft_plan_create_r2c_1d(None, 1024, 1) == 3
+    assert h.value is None
+    assert L.tcfft_exec_r2c(None, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), None) == 4
+    assert L.tcfft_exec_c2r(None, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), None) == 4

@@ -1,6 +1,6 @@
 """hipGraph capture of the entry points that allocate nothing: the drop-in
 exec (1D and 2D), out-of-place natural order, the uneven batch, split
-precision and the fused convolution.  Each graph is replayed three times with
+precision, the real-input transforms and the fused convolution.  Each graph is replayed three times with
 eager work (input restore, an eager transform of another buffer) between the
 replays, and every replay must equal the eager result bit for bit."""
 import sys
@@ -142,5 +142,32 @@ def test_graph_fused_convolution():
 
     replay_check(lambda st: plan.conv_exec(ch, x.data_ptr(), h.data_ptr(), y.data_ptr(), False, spec.data_ptr(), st),
                  reset, y, want, lambda: plan.exec_dif(x.data_ptr(), torch.empty_like(x).data_ptr()))
+    assert torch.equal(spec, spec_w)
+    plan.destroy()
+
+
+def test_graph_real_input():
+    note("[gpu] graph: r2c then c2r, n=8192 x 12")
+    rng = np.random.default_rng(11)
+    n, batch = 8192, 12
+    plan = tcfft_amd.PlanR2C(n, batch)
+    x = fp16(rng, (batch, n))
+    spec_w = torch.empty(batch, n // 2 + 1, 2, dtype=torch.float16, device="cuda")
+    back_w = torch.empty_like(x)
+    plan.exec_r2c(x.data_ptr(), spec_w.data_ptr())
+    plan.exec_c2r(spec_w.data_ptr(), back_w.data_ptr())
+    torch.cuda.synchronize()
+    spec, back = torch.empty_like(spec_w), torch.empty_like(x)
+
+    def reset():
+        spec.zero_()
+        back.zero_()
+
+    def launch(st):
+        plan.exec_r2c(x.data_ptr(), spec.data_ptr(), st)
+        plan.exec_c2r(spec.data_ptr(), back.data_ptr(), st)
+
+    replay_check(launch, reset, back, back_w,
+                 lambda: plan.exec_r2c(x.data_ptr(), torch.empty_like(spec_w).data_ptr()))
     assert torch.equal(spec, spec_w)
     plan.destroy()

@@ -269,3 +269,82 @@ def test_planted_mistakes_are_caught():
     assert oracle.rel_l2(two_d, ref) > 0.5
     # transposed digit order (natural input instead of digit-reversed)
     assert oracle.rel_l2(oracle.dit(x, rad)[0], ref) > 0.5
+
+
+# ---------------------------------------------------------------- real input
+@pytest.mark.parametrize("n", [16, 512, 8192])
+def test_rfft_irfft_match_numpy(n):
+    x = RNG.uniform(-1, 1, (3, n))
+    X = oracle.rfft(x)
+    assert X.shape == (3, n // 2 + 1)
+    assert oracle.rel_l2(X, np.fft.rfft(x, axis=-1)) < 1e-13
+    Y = crand((3, n // 2 + 1))
+    y = oracle.irfft(Y, n)
+    assert y.shape == (3, n)
+    np.testing.assert_allclose(y, n * np.fft.irfft(Y, n, axis=-1), rtol=0, atol=1e-10 * n)
+    # round trip and Hermitian symmetry of the full transform it truncates
+    np.testing.assert_allclose(oracle.irfft(X, n), n * x, rtol=0, atol=1e-9 * n)
+    full = oracle.fft(x.astype(np.complex128))
+    np.testing.assert_allclose(full[:, 1:][:, ::-1], np.conj(full[:, 1:]), rtol=0, atol=1e-9 * n)
+
+
+def test_rfft_closed_forms():
+    n, k0 = 1024, 37
+    t = np.arange(n)
+    want = np.zeros(n // 2 + 1, dtype=np.complex128)
+    want[k0] = n / 2
+    np.testing.assert_allclose(oracle.rfft(np.cos(2 * np.pi * k0 * t / n)[None])[0], want, atol=1e-10)
+    want = np.zeros(n // 2 + 1, dtype=np.complex128)
+    want[n // 2] = n
+    np.testing.assert_allclose(oracle.rfft(((-1.0) ** t)[None])[0], want, atol=1e-10)
+    # the imaginary parts of bins 0 and n/2 do not reach the inverse
+    Y = crand((1, n // 2 + 1))
+    Z = Y.copy()
+    Z[0, 0] += 5j
+    Z[0, -1] -= 3j
+    np.testing.assert_array_equal(oracle.irfft(Y, n), oracle.irfft(Z, n))
+
+
+def untangle(Z, sign=-1):
+    """The even/odd split the real-input kernels compute, restated in numpy:
+    X[k] = ((Z[k] + conj Z[L-k]) - i W^k (Z[k] - conj Z[L-k])) / 2."""
+    L = Z.shape[-1]
+    k = np.arange(L + 1)
+    Zk = Z[..., k % L]
+    Zm = np.conj(Z[..., (L - k) % L])
+    w = np.exp(sign * 2j * np.pi * k / (2 * L))
+    return ((Zk + Zm) - 1j * w * (Zk - Zm)) / 2
+
+
+def tangle(X):
+    """Inverse pre-step: Z'[k] = (X[k] + conj X[L-k]) + i W^-k (X[k] - conj X[L-k])."""
+    L = X.shape[-1] - 1
+    X = X.copy()
+    X[..., 0] = X[..., 0].real
+    X[..., L] = X[..., L].real
+    k = np.arange(L)
+    a, b = X[..., k], np.conj(X[..., L - k])
+    return (a + b) + 1j * np.exp(2j * np.pi * k / (2 * L)) * (a - b)
+
+
+@pytest.mark.parametrize("n", [512, 8192])
+def test_even_odd_split_against_oracle(n):
+    """The kernels' formulas against the oracle; a flipped twiddle sign, a
+    missing conjugate or a dropped factor 1/2 fails."""
+    x = RNG.uniform(-1, 1, (2, n))
+    z = x[:, 0::2] + 1j * x[:, 1::2]
+    Z = oracle.fft(z)
+    ref = oracle.rfft(x)
+    assert oracle.rel_l2(untangle(Z), ref) < 1e-13
+    assert oracle.rel_l2(untangle(Z, +1), ref) > 0.1
+    L = n // 2
+    k = np.arange(L + 1)
+    no_conj = ((Z[:, k % L] + Z[:, (L - k) % L]) - 1j * np.exp(-2j * np.pi * k / n)
+               * (Z[:, k % L] - Z[:, (L - k) % L])) / 2
+    assert oracle.rel_l2(no_conj, ref) > 0.1
+    assert oracle.rel_l2(2 * untangle(Z), ref) > 0.5
+    X = crand((2, L + 1))
+    y = oracle.irfft(X, n)
+    zz = oracle.fft(tangle(X), +1)
+    got = np.stack([zz.real, zz.imag], -1).reshape(2, n)
+    np.testing.assert_allclose(got, y, rtol=0, atol=1e-9 * n)
