@@ -13,6 +13,17 @@ backward of fft is the hand-written adjoint kernel (F^H) and the backward of
 fft_adjoint is the forward kernel (F).  No CPU fallback: these raise if the
 library or a GPU is missing.
 
+Real input (n = 512..8192):
+
+    rfft(x)          bins 0..n/2 of F x for real fp16 x [..., n] -> [..., n/2+1, 2]
+    c2r(X, n)        y = sum over the Hermitian extension of X, n * irfft
+    irfft(X, n)      c2r(X, n) / n (the scale is applied in fp32)
+
+Their backward passes are each other's kernels: viewed as real-linear maps,
+the transpose of rfft is c2r applied to the gradient with bins 1..n/2-1
+halved (c2r counts those bins twice), and the transpose of c2r is rfft of the
+gradient with bins 1..n/2-1 doubled.
+
 FFTConv1d implements y = F^H (H * F x) / n with a learnable filter spectrum H
 broadcast over leading dimensions (circular convolution by the impulse
 response F^H H / n); for n = 4096, 8192 and 16384 forward and backward run
@@ -23,9 +34,10 @@ from __future__ import annotations
 
 import torch
 
-from . import FORWARD, INVERSE, Plan1D, gen_rev
+from . import FORWARD, INVERSE, Plan1D, PlanR2C, gen_rev
 
 _PLANS: dict = {}
+_RPLANS: dict = {}
 
 
 def _plan(n: int, batch: int, device: torch.device) -> Plan1D:
@@ -91,6 +103,81 @@ def ifft(x: torch.Tensor) -> torch.Tensor:
     """Normalised inverse, F^H x / n (the scale is applied in fp32)."""
     n = x.shape[-2]
     return (fft_adjoint(x).float() / n).half()
+
+
+def _rplan(n: int, batch: int, device: torch.device) -> PlanR2C:
+    key = (n, batch, device.index)
+    p = _RPLANS.get(key)
+    if p is None:
+        with torch.cuda.device(device):
+            p = PlanR2C(n, batch)
+        _RPLANS[key] = p
+    return p
+
+
+def _r2c(x: torch.Tensor) -> torch.Tensor:
+    if x.device.type != "cuda" or x.dtype != torch.float16 or x.dim() < 1:
+        raise RuntimeError("expected a real fp16 CUDA (HIP) tensor of shape [..., n]")
+    x = x.contiguous()
+    n = x.shape[-1]
+    batch = x.numel() // n
+    out = torch.empty(x.shape[:-1] + (n // 2 + 1, 2), dtype=torch.float16, device=x.device)
+    _rplan(n, batch, x.device).exec_r2c(x.data_ptr(), out.data_ptr(), torch.cuda.current_stream(x.device).cuda_stream)
+    return out
+
+
+def _c2r(X: torch.Tensor, n: int) -> torch.Tensor:
+    _check(X)
+    if X.shape[-2] != n // 2 + 1:
+        raise RuntimeError(f"expected {n // 2 + 1} bins for n = {n}, got {X.shape[-2]}")
+    X = X.contiguous()
+    batch = X.numel() // (2 * (n // 2 + 1))
+    out = torch.empty(X.shape[:-2] + (n,), dtype=torch.float16, device=X.device)
+    _rplan(n, batch, X.device).exec_c2r(X.data_ptr(), out.data_ptr(), torch.cuda.current_stream(X.device).cuda_stream)
+    return out
+
+
+def _scale_interior(t: torch.Tensor, f: float) -> torch.Tensor:
+    v = t.float()
+    v[..., 1:-1, :] *= f
+    return v.half()
+
+
+class _RFFT(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        ctx.n = x.shape[-1]
+        return _r2c(x)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _c2r(_scale_interior(g, 0.5), ctx.n)
+
+
+class _C2R(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, X, n):
+        return _c2r(X, n)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _scale_interior(_r2c(g), 2.0), None
+
+
+def rfft(x: torch.Tensor) -> torch.Tensor:
+    """Differentiable real-input FFT: fp16 [..., n] -> bins 0..n/2 as [..., n/2+1, 2]."""
+    return _RFFT.apply(x)
+
+
+def c2r(X: torch.Tensor, n: int) -> torch.Tensor:
+    """Differentiable unnormalised Hermitian inverse (n times irfft); the
+    imaginary parts of bins 0 and n/2 are ignored."""
+    return _C2R.apply(X, n)
+
+
+def irfft(X: torch.Tensor, n: int) -> torch.Tensor:
+    """Normalised inverse of rfft, c2r(X, n) / n (the scale is applied in fp32)."""
+    return (c2r(X, n).float() / n).half()
 
 
 def complex_mul(a: torch.Tensor, h: torch.Tensor) -> torch.Tensor:

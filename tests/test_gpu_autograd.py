@@ -51,6 +51,42 @@ def test_fft_and_adjoint_values(shape):
 
 
 @pytest.mark.parametrize("n", [512, 8192])
+def test_rfft_c2r_values_and_gradients_vs_torch_fft(n):
+    """rfft / c2r / irfft and their backward passes against torch.fft's
+    autograd in float64 on the CPU, for the loss sum(out * w)."""
+    note(f"[gpu] rfft/c2r autograd n={n}")
+    g = torch.Generator().manual_seed(n)
+    x = ((torch.rand(3, n, generator=g) * 2 - 1).half()).cuda().requires_grad_(True)
+    w = torch.randn(3, n // 2 + 1, 2, generator=g)
+    X = torch_fft.rfft(x)
+    (X.float() * w.cuda()).sum().backward()
+    xd = x.detach().double().cpu().requires_grad_(True)
+    Xd = torch.view_as_real(torch.fft.rfft(xd, dim=-1))
+    (Xd * w.double()).sum().backward()
+    assert rel(to_c(X), torch.view_as_complex(Xd.detach()).numpy()) <= 1e-3
+    assert rel(x.grad.double().cpu().numpy(), xd.grad.numpy()) <= TOL
+    # c2r = n * irfft, and its gradient
+    Xh = ((torch.rand(3, n // 2 + 1, 2, generator=g) * 2 - 1).half()).cuda().requires_grad_(True)
+    v = torch.randn(3, n, generator=g)
+    y = torch_fft.c2r(Xh, n)
+    (y.float() * v.cuda()).sum().backward()
+    Xd = Xh.detach().double().cpu().requires_grad_(True)
+    yd = torch.fft.irfft(torch.view_as_complex(Xd), n=n, dim=-1) * n
+    (yd * v.double()).sum().backward()
+    assert rel(y.detach().double().cpu().numpy(), yd.detach().numpy()) <= 1e-3
+    # torch's irfft ignores the imaginary parts of bins 0 and n/2: so does c2r
+    gd = Xd.grad.clone()
+    gd[:, 0, 1] = 0
+    gd[:, -1, 1] = 0
+    gg = Xh.grad.double().cpu()
+    assert float(gg[:, 0, 1].abs().max()) == 0.0 and float(gg[:, -1, 1].abs().max()) == 0.0
+    assert rel(to_c(gg), to_c(gd)) <= TOL
+    # the normalised inverse round-trips
+    back = torch_fft.irfft(torch_fft.rfft(x.detach()), n)
+    assert rel(back.double().cpu().numpy(), x.detach().double().cpu().numpy()) <= 1.5e-3
+
+
+@pytest.mark.parametrize("n", [512, 8192])
 def test_fft_backward_matches_torch_fft(n):
     note(f"[gpu] fft backward n={n}")
     x = fp16_pair((4, n), 2).cuda().requires_grad_(True)
