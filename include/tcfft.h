@@ -113,9 +113,9 @@ int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, 
 
 /* Real-input transforms (no counterpart in the reference, whose callers feed
  * real data as complex with zero imaginary parts, e.g. the generator of
- * doit.h:7-19 fills both).  A plan for n_batch real signals of n = 512, 1024,
- * 2048, 4096 or 8192 points (else TCFFT_INVALID_SIZE); it is also the complex
- * plan of n/2 points for every other entry point.  Each direction is one HBM
+ * doit.h:7-19 fills both).  A plan for n_batch real signals of n = 2^9 ..
+ * 2^16 points (else TCFFT_INVALID_SIZE); it is also the complex plan of n/2
+ * points for every other entry point.  Each direction is one HBM
  * round trip of the half-size data:
  *   tcfft_exec_r2c: in = n_batch * n FP16 reals (16-byte aligned), out =
  *     bins 0..n/2 of each signal's DFT (sign e^{-2 pi i nk/N}), n/2 + 1 FP16

@@ -317,6 +317,47 @@ __device__ __forceinline__ void big_phase2(u32 (&e)[16][4], int w, u32* lds, u32
     }
 }
 
+// big_phase2 for W = 2, 4, 8 that keeps the results in registers instead of
+// storing them: o[(r*QN + q)*W + s] holds the four words of positions
+// 2048r + 4*lane + 256*(q + QN*w) + 4096*s + (0..3) (the words big_phase2
+// stores there).  The caller synchronises before reusing the LDS.
+template <int W, int ROUNDS, int SIGN>
+__device__ __forceinline__ void big_phase2_regs(u32 (&e)[16][4], int w, u32* lds, u32 (&o)[16][4]) {
+    static_assert(W >= 2 && W <= 8, "VALU radix-W stage");
+    using C = BigCfg<W, ROUNDS>;
+    constexpr int QN = C::COLS / (256 * W);
+    static_assert(ROUNDS * QN * W == 16, "one slot per output group");
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int r = 0; r < C::ROUNDS; ++r) {
+        constexpr int TPR = 16 / C::ROUNDS;
+        if (r) __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < TPR; ++tt) {
+            const u32 off = (u32)w * C::COLS + lds_sigma(4u * g + 16u * c16 + 256u * tt);
+            store4(lds + off, e[r * TPR + tt]);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < QN; ++q) {
+            u32 v[W][4];
+#pragma unroll
+            for (int s = 0; s < W; ++s)
+                load4(v[s], lds + (u32)s * C::COLS + lds_sigma(4u * lane + 256u * (q + QN * w)));
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                cf x[W];
+#pragma unroll
+                for (int s = 0; s < W; ++s) x[s] = unpack(v[s][j]);
+                small_dft<W>(x, SIGN);
+#pragma unroll
+                for (int s = 0; s < W; ++s) o[(r * QN + q) * W + s][j] = pack(x[s]);
+            }
+        }
+    }
+}
+
 // One signal of N = 4096*W at sbase, processed by the W waves of a group
 // (this wave is number w) with lds = the group's W*COLS words of LDS.  sig
 // is the signal's index (epilogue only).

@@ -925,6 +925,146 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
 }
 
 // ----------------------------------------------------------------------------
+// Real-input transforms of N = 8192*W points (W = 2, 4, 8) on k_big_nat: the
+// same even/odd split as k_r2c / k_c2r (see there) over L = 4096*W complex
+// points, one workgroup of W waves per signal.  Wave w handles the pairs
+// k = 64*(j*W + w) + lane, j < 32; W_N^k = wn[64w + lane] * wn[64W]^j (wn =
+// W_N^0..64W).
+//
+// k_r2c_big<W>: the k_big_nat pass with the final results kept in registers
+// (big_phase2_regs), staged in natural order over the whole LDS (4096*W
+// words, lane-contiguous 16-byte writes), then the untangle.
+template <int W>
+__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_r2c_big(
+    const u32* __restrict__ in, u32* __restrict__ out, const float2* __restrict__ w4096,
+    const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo, u32 wn_lo_bits,
+    const float2* __restrict__ wn, StartSched sched) {
+    extern __shared__ u32 lds[];
+    constexpr u32 L = 4096u * W;
+    start_delay(sched);
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+    const u32 w = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 sig = blockIdx.x;
+    BigTw bt;
+    big_tw_load<W, false>(bt, lane, (int)w, sig, w4096, wn_hi, wn_lo, wn_lo_bits, EpiParams{});
+    __builtin_amdgcn_sched_barrier(0);
+    u32 d[16][4];
+    {
+        u32 v[16][4];
+        const u32* src = in + (size_t)sig * L + 4096u * w + 4u * lane;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) load4_nt(v[t], src + 256u * t);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) store4(lds + nat_big_word<W>(4096u * w + 256u * t + 4u * lane), v[t]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            d[t][j] = lds[nat_big_word<W>(w + W * ((u32)t + 16u * c16 + 256u * (4u * g + j)))];
+    __syncthreads();
+    const Dft16Op op = make_dft16(lane, -1);
+    u32 e[16][4];
+    big_phase1<-1>(d, e, bt, op);
+    u32 o[16][4];
+    big_phase2_regs<W, BigCfg<W>::ROUNDS, -1>(e, (int)w, lds, o);
+    const float2 wb = wn[64u * w + (u32)lane], ws = wn[64u * W];
+    __syncthreads();  // every wave is done with the exchange
+    constexpr int QN = BigCfg<W>::COLS / (256 * W);
+#pragma unroll
+    for (int r = 0; r < BigCfg<W>::ROUNDS; ++r)
+#pragma unroll
+        for (int q = 0; q < QN; ++q)
+#pragma unroll
+            for (int s = 0; s < W; ++s)
+                store4(lds + 2048u * r + 4u * lane + 256u * (q + QN * w) + 4096u * s, o[(r * QN + q) * W + s]);
+    __syncthreads();
+    u32* const dst = out + (size_t)sig * (L + 1u);
+    const cf step = {ws.x, ws.y};
+    cf tw = {0.5f * wb.x, 0.5f * wb.y};  // W_N^k / 2
+#pragma unroll 8
+    for (u32 j = 0; j < 32u; ++j) {
+        const u32 k = 64u * (j * W + w) + (u32)lane;  // < L/2
+        const cf a = unpack(lds[k]);
+        const cf b = unpack(lds[(L - k) & (L - 1u)]);
+        const cf hs = {0.5f * (a.x + b.x), 0.5f * (a.y - b.y)};
+        const cf dif = {a.x - b.x, a.y + b.y};
+        const cf q = cmul(tw, dif);
+        dst[k] = pack(hs.x + q.y, hs.y - q.x);
+        dst[L - k] = pack(hs.x - q.y, -(hs.y + q.x));
+        tw = cmul(tw, step);
+    }
+    if (w == 0 && lane == 0) {
+        const cf a = unpack(lds[L / 2u]);
+        dst[L / 2u] = pack(a.x, -a.y);
+    }
+}
+
+// k_c2r_big<W>: Z' formed pairwise from the bins into the LDS staging of
+// k_big_nat (its swizzle), then the k_big_nat inverse pass to the real output.
+template <int W>
+__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_c2r_big(
+    const u32* __restrict__ in, u32* __restrict__ out, const float2* __restrict__ w4096,
+    const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo, u32 wn_lo_bits,
+    const float2* __restrict__ wn, StartSched sched) {
+    extern __shared__ u32 lds[];
+    constexpr u32 L = 4096u * W;
+    start_delay(sched);
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+    const u32 w = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 sig = blockIdx.x;
+    BigTw bt;
+    big_tw_load<W, false>(bt, lane, (int)w, sig, w4096, wn_hi, wn_lo, wn_lo_bits, EpiParams{});
+    const float2 wb = wn[64u * w + (u32)lane], ws = wn[64u * W];
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const u32* const src = in + (size_t)sig * (L + 1u);
+        u32 va[32], vb[32];
+#pragma unroll
+        for (u32 j = 0; j < 32u; ++j) {
+            const u32 k = 64u * (j * W + w) + (u32)lane;
+            va[j] = __builtin_nontemporal_load(src + k);
+            vb[j] = __builtin_nontemporal_load(src + (L - k));
+        }
+        const u32 vm = (w == 0 && lane == 0) ? src[L / 2u] : 0u;
+        const cf step = {ws.x, -ws.y};
+        cf tw = {wb.x, -wb.y};  // W_N^-k
+#pragma unroll
+        for (u32 j = 0; j < 32u; ++j) {
+            const u32 k = 64u * (j * W + w) + (u32)lane;
+            cf a = unpack(va[j]), b = unpack(vb[j]);
+            if (k == 0) a.y = b.y = 0.f;  // bins 0 and L: real parts only
+            const cf sum = {a.x + b.x, a.y - b.y};
+            const cf dif = {a.x - b.x, a.y + b.y};
+            const cf q = cmul(tw, dif);
+            lds[nat_big_word<W>(k)] = pack(sum.x - q.y, sum.y + q.x);
+            if (k != 0) lds[nat_big_word<W>(L - k)] = pack(sum.x + q.y, q.x - sum.y);
+            tw = cmul(tw, step);
+        }
+        if (w == 0 && lane == 0) {
+            const cf a = unpack(vm);
+            lds[nat_big_word<W>(L / 2u)] = pack(2.f * a.x, -2.f * a.y);
+        }
+    }
+    __syncthreads();
+    u32 d[16][4];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            d[t][j] = lds[nat_big_word<W>(w + W * ((u32)t + 16u * c16 + 256u * (4u * g + j)))];
+    __syncthreads();
+    const Dft16Op op = make_dft16(lane, +1);
+    u32 e[16][4];
+    big_phase1<+1>(d, e, bt, op);
+    big_phase2<W, BigCfg<W>::ROUNDS, false, +1>(e, (int)w, lds, out + (size_t)sig * L, bt, op, true);
+}
+
+// ----------------------------------------------------------------------------
 // Host-side launchers (called by the pass driver).
 // One-time set-up of a kernel, cached by the kernel's address (every
 // template instance of a kernel has the same pointer type, so a static per
@@ -1047,6 +1187,32 @@ static hipError_t launch_r2c_t(int sign, const u32* in, u32* out, u32 n_tiles, c
     const u32 waves = (n_tiles + 15) / 16;
     hipLaunchKernelGGL(k, dim3((waves + 3) / 4), dim3(256), lds, st, in, out, n_tiles, w4096, wn);
     return hipGetLastError();
+}
+
+template <int W>
+static hipError_t launch_r2c_big_t(int sign, const u32* in, u32* out, u32 n_signals, const float2* w4096,
+                                   const float2* wn_hi, const float2* wn_lo, u32 lo_bits, const float2* wn,
+                                   StartSched sched, hipStream_t st) {
+    const size_t lds = 4096u * W * sizeof(u32);  // the whole half-size signal
+    static_assert(4096u * W >= (size_t)BigCfg<W>::LDS_WORDS, "LDS covers the exchange");
+    auto k = sign < 0 ? k_r2c_big<W> : k_c2r_big<W>;
+    const KernelInfo ki = kernel_info(k, 64 * W, lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    if (sched.groups == 0) sched.delay = 0;
+    if (sched.delay) sched.resident = ki.resident;
+    hipLaunchKernelGGL(k, dim3(n_signals), dim3(64 * W), lds, st, in, out, w4096, wn_hi, wn_lo, lo_bits, wn, sched);
+    return hipGetLastError();
+}
+
+hipError_t launch_r2c_big(int W, int sign, const u32* in, u32* out, u32 n_signals, const float2* w4096,
+                          const float2* wn_hi, const float2* wn_lo, u32 lo_bits, const float2* wn, StartSched sched,
+                          hipStream_t st) {
+    switch (W) {
+    case 2: return launch_r2c_big_t<2>(sign, in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, wn, sched, st);
+    case 4: return launch_r2c_big_t<4>(sign, in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, wn, sched, st);
+    case 8: return launch_r2c_big_t<8>(sign, in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, wn, sched, st);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 hipError_t launch_r2c(int T, int sign, const u32* in, u32* out, u32 n_tiles, const float2* w4096, const float2* wn,

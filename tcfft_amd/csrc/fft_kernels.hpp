@@ -102,6 +102,12 @@ hipError_t launch_rows_nat(int T, const uint32_t* in, uint32_t* out, uint32_t n_
 // 0..N/2 out (batch * (N/2 + 1) words); sign +1: the inverse.  wn = W_N^0..64.
 hipError_t launch_r2c(int T, int sign, const uint32_t* in, uint32_t* out, uint32_t n_tiles, const float2* w4096,
                       const float2* wn, hipStream_t st);
+// Same for N = 8192*W (W = 2, 4, 8), one workgroup per signal; wn_hi/wn_lo/
+// lo_bits = the two-level W_{4096W} table of the half-size plan's first pass,
+// wn = W_N^0..64W.
+hipError_t launch_r2c_big(int W, int sign, const uint32_t* in, uint32_t* out, uint32_t n_signals,
+                          const float2* w4096, const float2* wn_hi, const float2* wn_lo, uint32_t lo_bits,
+                          const float2* wn, StartSched sched, hipStream_t st);
 
 // Tile of the LDS-tiled 1D digit reversal: a low bits of the position and
 // the b position bits in_bit[0..b-1] that form the low bits of the input

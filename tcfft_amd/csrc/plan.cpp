@@ -576,10 +576,32 @@ int tcfft_exec(tcfft_plan plan, void* data, int direction, void* stream) {
     return run_passes(plan, (uint32_t*)data, direction, (hipStream_t)stream);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Real-input kernels of a real-input plan: the row kernels for n <= 8192, the
+// workgroup kernels (the half-size plan's 4096W-point first pass and its
+// twiddle tables) for n = 16384..65536.
+hipError_t launch_real(tcfft_plan_s* p, int sign, const void* in, void* out, hipStream_t st) {
+    const uint32_t half = (uint32_t)p->real_n / 2, batch = (uint32_t)p->lay.batch;
+    if (half <= 4096)
+        return tcfft::launch_r2c((int)(half / 256), sign, (const uint32_t*)in, (uint32_t*)out, batch * (half / 256),
+                                 p->w4096, p->w_real, st);
+    const Pass& ps = p->lay.passes[0];
+    if (ps.kind != 2 || ps.radix != (int)(half / 4096)) return hipErrorInvalidValue;
+    return tcfft::launch_r2c_big(ps.radix, sign, (const uint32_t*)in, (uint32_t*)out, batch, p->w4096, ps.big_hi,
+                                 ps.big_lo, ps.big_M_log2 / 2, p->w_real, big_sched(p, ps.radix), st);
+}
+
+}  // namespace
+
+extern "C" {
+
 int tcfft_plan_create_r2c_1d(tcfft_plan* plan, int n, int n_batch) {
     if (!plan) return TCFFT_INVALID_POINTER;
     *plan = nullptr;
-    if (!is_pow2(n) || n < 512 || n > 8192) return TCFFT_INVALID_SIZE;
+    if (!is_pow2(n) || n < 512 || n > 65536) return TCFFT_INVALID_SIZE;
     Layout lay;
     int rc = build_layout_1d(n / 2, n_batch, lay);
     if (rc) return rc;
@@ -587,7 +609,9 @@ int tcfft_plan_create_r2c_1d(tcfft_plan* plan, int n, int n_batch) {
     if (rc) return rc;
     tcfft_plan_s* p = *plan;
     p->real_n = n;
-    rc = upload(p, host_table((uint32_t)ilog2((uint32_t)n), 1, 65), &p->w_real);
+    // W_n^0..64 for the row kernels, W_n^0..64W for the workgroup kernels (n = 8192 W)
+    const uint32_t count = n <= 8192 ? 65u : 64u * (uint32_t)(n / 8192) + 1u;
+    rc = upload(p, host_table((uint32_t)ilog2((uint32_t)n), 1, count), &p->w_real);
     if (rc) {
         free_plan(p);
         *plan = nullptr;
@@ -603,8 +627,7 @@ int tcfft_exec_r2c(tcfft_plan plan, const void* in, void* out, void* stream) {
     if (!in || !out || (a & 15u) || (b & 3u)) return TCFFT_INVALID_POINTER;
     if (a < b + (half + 1) * batch * 4u && b < a + half * batch * 4u) return TCFFT_INVALID_POINTER;  // overlap
     DeviceGuard guard(plan->device);
-    hipError_t e = tcfft::launch_r2c((int)(half / 256), -1, (const uint32_t*)in, (uint32_t*)out,
-                                     (uint32_t)(batch * (half / 256)), plan->w4096, plan->w_real, (hipStream_t)stream);
+    hipError_t e = launch_real(plan, -1, in, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
     return TCFFT_SUCCESS;
 }
@@ -617,8 +640,7 @@ int tcfft_exec_c2r(tcfft_plan plan, const void* in, void* out, void* stream) {
     if (!in || !out || (a & 3u) || (b & 15u)) return TCFFT_INVALID_POINTER;
     if (a < b + half * batch * 4u && b < a + (half + 1) * batch * 4u) return TCFFT_INVALID_POINTER;  // overlap
     DeviceGuard guard(plan->device);
-    hipError_t e = tcfft::launch_r2c((int)(half / 256), +1, (const uint32_t*)in, (uint32_t*)out,
-                                     (uint32_t)(batch * (half / 256)), plan->w4096, plan->w_real, (hipStream_t)stream);
+    hipError_t e = launch_real(plan, +1, in, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
     return TCFFT_SUCCESS;
 }

@@ -1,7 +1,8 @@
 """Real-input transforms (tcfft_exec_r2c / tcfft_exec_c2r) against the FP64
 oracle: bins 0..n/2 of the DFT of real fp16 signals and the unnormalised
-Hermitian inverse, n = 512..8192, ragged batches (a wave holds 16*256/(n/2)
-signals), the reference's own glibc inputs, a round trip, agreement with the
+Hermitian inverse, n = 512..65536 (row kernels up to 8192, one workgroup per
+signal above), ragged batches (a row-kernel wave holds 4096/(n/2) signals),
+the reference's own glibc inputs, a round trip, agreement with the
 complex transform of the same data fed as complex, and the error paths.
 Tolerance: rel-L2 <= 1e-3 (round trip 1.5e-3)."""
 import sys
@@ -15,7 +16,7 @@ import tcfft_amd
 torch = pytest.importorskip("torch")
 pytestmark = pytest.mark.gpu
 
-SIZES = [512, 1024, 2048, 4096, 8192]
+SIZES = [512, 1024, 2048, 4096, 8192, 16384, 32768, 65536]
 
 
 def note(m):
@@ -86,7 +87,7 @@ def test_c2r_matches_oracle(n, batch):
     plan.destroy()
 
 
-@pytest.mark.parametrize("n", [1024, 8192])
+@pytest.mark.parametrize("n", [1024, 8192, 16384])
 def test_r2c_reference_inputs_and_round_trip(n):
     """The reference's glibc generator (doit.h:7-19), real parts as the real
     signal; then c2r(r2c(x)) = n x."""
@@ -103,7 +104,7 @@ def test_r2c_reference_inputs_and_round_trip(n):
     plan.destroy()
 
 
-@pytest.mark.parametrize("n", [2048, 8192])
+@pytest.mark.parametrize("n", [2048, 8192, 32768])
 def test_r2c_agrees_with_complex_transform_of_real_data(n):
     batch = 8
     rng = np.random.default_rng(n)
@@ -138,4 +139,19 @@ def test_r2c_error_paths():
         tcfft_amd._check(tcfft_amd.lib().tcfft_exec_r2c(cp._h, tcfft_amd._vp(p), tcfft_amd._vp(p + 65536), None),
                          "tcfft_exec_r2c")
     cp.destroy()
+    plan.destroy()
+
+
+@pytest.mark.parametrize("n", [16384, 65536])
+def test_r2c_large_round_trip(n):
+    """c2r(r2c(x)) = n x for the workgroup kernels (x scaled so that n x
+    stays inside fp16's range)."""
+    batch = 6
+    rng = np.random.default_rng(n + 5)
+    x = (rng.uniform(-1, 1, (batch, n)) * (0.25 if n > 32768 else 1.0)).astype(np.float16).astype(np.float64)
+    plan = tcfft_amd.PlanR2C(n, batch)
+    back = c2r(plan, r2c(plan, x), n) / n
+    err = float(np.linalg.norm(back - x) / np.linalg.norm(x))
+    note(f"[gpu] r2c round trip n={n}: rel_l2={err:.3e}")
+    assert err <= 1.5e-3
     plan.destroy()
