@@ -554,6 +554,44 @@ def hipfft_fp16(torch, dims, batch, reps):
     return out
 
 
+def hipfft_r2c_fp16(torch, n, batch, reps):
+    """hipFFT's half-precision R2C on the same real workload
+    (hipfftXtMakePlanMany, HIP_R_16F -> HIP_C_16F, out of place), eager."""
+    import ctypes
+    try:
+        lib = ctypes.CDLL("/opt/rocm/lib/libhipfft.so")
+    except OSError as e:
+        return {"error": f"libhipfft not loadable: {e}"}
+    HIP_R_16F, HIP_C_16F = 2, 6
+    h = ctypes.c_void_p()
+    if lib.hipfftCreate(ctypes.byref(h)) != 0:
+        return {"error": "hipfftCreate failed"}
+    dims = (ctypes.c_longlong * 1)(n)
+    bins = n // 2 + 1
+    ws = ctypes.c_size_t()
+    rc = lib.hipfftXtMakePlanMany(h, 1, dims, None, ctypes.c_longlong(1), ctypes.c_longlong(n), HIP_R_16F, None,
+                                  ctypes.c_longlong(1), ctypes.c_longlong(bins), HIP_C_16F, ctypes.c_longlong(batch),
+                                  ctypes.byref(ws), HIP_C_16F)
+    if rc != 0:
+        lib.hipfftDestroy(h)
+        return {"error": f"hipfftXtMakePlanMany (R2C) returned {rc}"}
+    lib.hipfftSetStream(h, ctypes.c_void_p(torch.cuda.current_stream().cuda_stream))
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-1, 1, (batch, n)).astype(np.float16)
+    src = rotation(torch, (batch, n), torch.float16, torch.from_numpy(x))
+    dst = torch.empty(batch, bins, 2, dtype=torch.float16, device="cuda")
+    lib.hipfftXtExec(h, ctypes.c_void_p(src[0].data_ptr()), ctypes.c_void_p(dst.data_ptr()), -1)
+    torch.cuda.synchronize()
+    k = min(batch, 2)
+    err = rel(to_c(dst[:k]), np.fft.rfft(x[:k].astype(np.float64), axis=-1))
+    ns = len(src)
+    ms = eager_time(torch, lambda i: lib.hipfftXtExec(h, ctypes.c_void_p(src[i % ns].data_ptr()),
+                                                       ctypes.c_void_p(dst.data_ptr()), -1), reps)
+    lib.hipfftDestroy(h)
+    return {"ms": round(ms, 5), "rel_l2_vs_fp64": err, "work_bytes": int(ws.value),
+            "library": "hipFFT/rocFFT (/opt/rocm), HIP_R_16F -> HIP_C_16F, out of place, eager"}
+
+
 def stream_copy(torch):
     """Device-to-device copy bandwidth over a > MALL rotation (TB/s, read+write)."""
     src = rotation(torch, (64 << 20,), torch.float32, torch.zeros(64 << 20))  # 256 MiB each
@@ -594,7 +632,8 @@ def main():
         leg = {"split": lambda: bench_split(torch, tcfft_amd, args.steps, args.split_chunk),
                "uneven": lambda: bench_uneven(torch, tcfft_amd, args.steps),
                "conv": lambda: bench_conv(torch, max(3, args.steps // 20)),
-               "r2c": lambda: bench_r2c(torch, tcfft_amd, args.steps)}[args.workload]()
+               "r2c": lambda: {**bench_r2c(torch, tcfft_amd, args.steps),
+                               "hipfft_r2c": hipfft_r2c_fp16(torch, 8192, 64 * 256, args.steps)}}[args.workload]()
         print(json.dumps({"workload": args.workload, **leg}), flush=True)
         return
 
@@ -682,6 +721,7 @@ def main():
         cfg["hipfft_fp16_n2^16_b256"] = hipfft_fp16(torch, [n], batch, reps)
         cfg["hipfft_fp16_n4096_b1024"] = hipfft_fp16(torch, [4096], 1024, reps)
         cfg["hipfft_fp16_2d_1024x1024_b8"] = hipfft_fp16(torch, [1024, 1024], 8, reps)
+        cfg["hipfft_r2c_fp16_n8192_16384_signals"] = hipfft_r2c_fp16(torch, 8192, 64 * 256, reps)
         cfg["stream_copy_tbs"] = stream_copy(torch)
         out["configs"] = cfg
     if not args.no_cpu:
