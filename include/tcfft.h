@@ -114,9 +114,13 @@ int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, 
 /* Real-input transforms (no counterpart in the reference, whose callers feed
  * real data as complex with zero imaginary parts, e.g. the generator of
  * doit.h:7-19 fills both).  A plan for n_batch real signals of n = 2^9 ..
- * 2^16 points (else TCFFT_INVALID_SIZE); it is also the complex plan of n/2
- * points for every other entry point.  Each direction is one HBM
- * round trip of the half-size data:
+ * 2^27 points (else TCFFT_INVALID_SIZE); it is also the complex plan of n/2
+ * points for every other entry point.  Up to n = 2^16 each direction is one
+ * HBM round trip of the half-size data (the even/odd split fused into the
+ * transform).  Above, the n/2-point transform runs as the plan's passes in
+ * natural order through a workspace of n * n_batch * 2 bytes that the plan
+ * allocates at creation, and the split is a pass of its own; concurrent
+ * calls on one such plan race on that workspace.
  *   tcfft_exec_r2c: in = n_batch * n FP16 reals (16-byte aligned), out =
  *     bins 0..n/2 of each signal's DFT (sign e^{-2 pi i nk/N}), n/2 + 1 FP16
  *     complex per signal, signal b at out + 4*(n/2+1)*b bytes (4-byte aligned);
@@ -125,7 +129,7 @@ int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, 
  *     ignored), out = y[t] = sum_{k<n} X[k] e^{+2 pi i kt/n} (unnormalised,
  *     n times numpy's irfft), n FP16 reals per signal.
  * Out of place only (the buffers must not overlap: TCFFT_INVALID_POINTER);
- * no allocation, graph-capturable.  A plan not made by
+ * no allocation in the exec calls, graph-capturable.  A plan not made by
  * tcfft_plan_create_r2c_1d gives TCFFT_INVALID_PLAN. */
 int tcfft_plan_create_r2c_1d(tcfft_plan *plan, int n, int n_batch);
 int tcfft_exec_r2c(tcfft_plan plan, const void *in, void *out, void *stream);

@@ -279,7 +279,7 @@ class Plan1D(_Plan):
 
 
 class PlanR2C(_Plan):
-    """Real-input transforms of batch signals of n = 512..65536 real points
+    """Real-input transforms of batch signals of n = 2^9..2^27 real points
     (no reference counterpart).  exec_r2c: n fp16 reals -> bins 0..n/2
     (n/2 + 1 fp16 complex per signal); exec_c2r: the unnormalised inverse
     (n times numpy's irfft).  Out of place.  As a complex plan (exec,

@@ -925,6 +925,56 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
 }
 
 // ----------------------------------------------------------------------------
+// Real-input transforms above 65536 points: the L = N/2-point complex
+// transform runs as its own passes and k_pairs<SIGN> is the even/odd split by
+// itself, one lane per bin pair (k, L-k), k < L/2 (the lane with k = 0 also
+// takes bin L/2), with W_N^k = hi[k >> lo_bits] * lo[k & mask] from the
+// two-level table of W_N (same formulas as k_r2c / k_c2r).
+//   SIGN -1: Z (natural, L words per signal) -> bins 0..L (L+1 words per signal)
+//   SIGN +1: bins 0..L -> Z' (L words per signal)
+template <int SIGN>
+__global__ __launch_bounds__(256) void k_pairs(const u32* __restrict__ src, u32* __restrict__ dst, u32 l_log2,
+                                               u32 n_pairs, const float2* __restrict__ hi,
+                                               const float2* __restrict__ lo, u32 lo_bits) {
+    const u32 t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= n_pairs) return;
+    const u32 L = 1u << l_log2, half = L >> 1;
+    const size_t s = t >> (l_log2 - 1u);
+    const u32 k = t & (half - 1u);
+    const float2 th = hi[k >> lo_bits], tl = lo[k & ((1u << lo_bits) - 1u)];
+    const cf w = cmul(cf{th.x, th.y}, cf{tl.x, tl.y});  // W_N^k
+    if constexpr (SIGN < 0) {
+        const u32* const z = src + s * L;
+        u32* const x = dst + s * (L + 1u);
+        const cf a = unpack(z[k]);
+        const cf b = unpack(z[(L - k) & (L - 1u)]);
+        const cf hs = {0.5f * (a.x + b.x), 0.5f * (a.y - b.y)};  // S/2
+        const cf dif = {a.x - b.x, a.y + b.y};                   // D
+        const cf q = cmul(cf{0.5f * w.x, 0.5f * w.y}, dif);      // P = i q
+        x[k] = pack(hs.x + q.y, hs.y - q.x);
+        x[L - k] = pack(hs.x - q.y, -(hs.y + q.x));
+        if (k == 0) {
+            const cf m = unpack(z[half]);
+            x[half] = pack(m.x, -m.y);
+        }
+    } else {
+        const u32* const x = src + s * (L + 1u);
+        u32* const z = dst + s * L;
+        cf a = unpack(x[k]), b = unpack(x[L - k]);
+        if (k == 0) a.y = b.y = 0.f;  // bins 0 and L: real parts only
+        const cf sum = {a.x + b.x, a.y - b.y};
+        const cf dif = {a.x - b.x, a.y + b.y};
+        const cf q = cmul(cf{w.x, -w.y}, dif);  // W_N^-k D; P = i q
+        z[k] = pack(sum.x - q.y, sum.y + q.x);
+        if (k != 0) z[L - k] = pack(sum.x + q.y, q.x - sum.y);
+        if (k == 0) {
+            const cf m = unpack(x[half]);
+            z[half] = pack(2.f * m.x, -2.f * m.y);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // Real-input transforms of N = 8192*W points (W = 2, 4, 8) on k_big_nat: the
 // same even/odd split as k_r2c / k_c2r (see there) over L = 4096*W complex
 // points, one workgroup of W waves per signal.  Wave w handles the pairs
@@ -1213,6 +1263,16 @@ hipError_t launch_r2c_big(int W, int sign, const u32* in, u32* out, u32 n_signal
     case 8: return launch_r2c_big_t<8>(sign, in, out, n_signals, w4096, wn_hi, wn_lo, lo_bits, wn, sched, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_real_pairs(int sign, const u32* in, u32* out, u32 l_log2, u32 n_signals, const float2* hi,
+                             const float2* lo, u32 lo_bits, hipStream_t st) {
+    if (l_log2 < 2 || l_log2 > 26) return hipErrorInvalidValue;
+    const u32 pairs = n_signals << (l_log2 - 1u);
+    const u32 blocks = (pairs + 255u) / 256u;
+    if (sign < 0) hipLaunchKernelGGL(k_pairs<-1>, dim3(blocks), dim3(256), 0, st, in, out, l_log2, pairs, hi, lo, lo_bits);
+    else hipLaunchKernelGGL(k_pairs<+1>, dim3(blocks), dim3(256), 0, st, in, out, l_log2, pairs, hi, lo, lo_bits);
+    return hipGetLastError();
 }
 
 hipError_t launch_r2c(int T, int sign, const u32* in, u32* out, u32 n_tiles, const float2* w4096, const float2* wn,

@@ -102,6 +102,12 @@ hipError_t launch_rows_nat(int T, const uint32_t* in, uint32_t* out, uint32_t n_
 // 0..N/2 out (batch * (N/2 + 1) words); sign +1: the inverse.  wn = W_N^0..64.
 hipError_t launch_r2c(int T, int sign, const uint32_t* in, uint32_t* out, uint32_t n_tiles, const float2* w4096,
                       const float2* wn, hipStream_t st);
+// The even/odd split alone for N = 2L > 65536 (L = 2^l_log2): sign -1 maps
+// the natural L-point spectrum Z (L words per signal) to bins 0..L (L+1 words
+// per signal), sign +1 the bins to Z' (L words per signal).  hi/lo/lo_bits:
+// the two-level table of W_N.
+hipError_t launch_real_pairs(int sign, const uint32_t* in, uint32_t* out, uint32_t l_log2, uint32_t n_signals,
+                             const float2* hi, const float2* lo, uint32_t lo_bits, hipStream_t st);
 // Same for N = 8192*W (W = 2, 4, 8), one workgroup per signal; wn_hi/wn_lo/
 // lo_bits = the two-level W_{4096W} table of the half-size plan's first pass,
 // wn = W_N^0..64W.

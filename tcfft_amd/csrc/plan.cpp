@@ -206,7 +206,9 @@ struct tcfft_plan_s {
     int big_groups = 3;
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
     int real_n = 0;       // real-input plan: the real length (its complex layout has n = real_n / 2)
-    float2* w_real = nullptr;  // W_{real_n}^0..64
+    float2* w_real = nullptr;  // W_{real_n}^0..64 (n <= 8192) or ^0..64W (n = 8192 W <= 65536)
+    std::pair<float2*, float2*> w_real_pair{};  // n > 65536: two-level table of W_n
+    uint32_t* real_ws = nullptr;                // n > 65536: the n/2-point spectrum between the two steps
 };
 
 namespace {
@@ -601,7 +603,7 @@ extern "C" {
 int tcfft_plan_create_r2c_1d(tcfft_plan* plan, int n, int n_batch) {
     if (!plan) return TCFFT_INVALID_POINTER;
     *plan = nullptr;
-    if (!is_pow2(n) || n < 512 || n > 65536) return TCFFT_INVALID_SIZE;
+    if (!is_pow2(n) || n < 512 || n > (1 << 27)) return TCFFT_INVALID_SIZE;
     Layout lay;
     int rc = build_layout_1d(n / 2, n_batch, lay);
     if (rc) return rc;
@@ -609,9 +611,22 @@ int tcfft_plan_create_r2c_1d(tcfft_plan* plan, int n, int n_batch) {
     if (rc) return rc;
     tcfft_plan_s* p = *plan;
     p->real_n = n;
-    // W_n^0..64 for the row kernels, W_n^0..64W for the workgroup kernels (n = 8192 W)
-    const uint32_t count = n <= 8192 ? 65u : 64u * (uint32_t)(n / 8192) + 1u;
-    rc = upload(p, host_table((uint32_t)ilog2((uint32_t)n), 1, count), &p->w_real);
+    if (n <= 65536) {
+        // W_n^0..64 for the row kernels, W_n^0..64W for the workgroup kernels (n = 8192 W)
+        const uint32_t count = n <= 8192 ? 65u : 64u * (uint32_t)(n / 8192) + 1u;
+        rc = upload(p, host_table((uint32_t)ilog2((uint32_t)n), 1, count), &p->w_real);
+    } else {
+        rc = table_pair(p, (uint32_t)ilog2((uint32_t)n), p->w_real_pair);
+        if (!rc) {
+            void* ws = nullptr;
+            const hipError_t e = hipMalloc(&ws, (size_t)n / 2 * (size_t)n_batch * 4u);
+            if (e != hipSuccess) rc = hip_fail(e);
+            else {
+                p->allocs.push_back(ws);
+                p->real_ws = (uint32_t*)ws;
+            }
+        }
+    }
     if (rc) {
         free_plan(p);
         *plan = nullptr;
@@ -627,6 +642,16 @@ int tcfft_exec_r2c(tcfft_plan plan, const void* in, void* out, void* stream) {
     if (!in || !out || (a & 15u) || (b & 3u)) return TCFFT_INVALID_POINTER;
     if (a < b + (half + 1) * batch * 4u && b < a + half * batch * 4u) return TCFFT_INVALID_POINTER;  // overlap
     DeviceGuard guard(plan->device);
+    if (plan->real_ws) {
+        // n > 65536: the n/2-point transform of z into the workspace, then the split
+        int rc = tcfft_exec_natural(plan, in, plan->real_ws, TCFFT_FORWARD, stream);
+        if (rc) return rc;
+        const hipError_t e = tcfft::launch_real_pairs(-1, plan->real_ws, (uint32_t*)out,
+                                                      (uint32_t)ilog2((uint32_t)half), (uint32_t)batch,
+                                                      plan->w_real_pair.first, plan->w_real_pair.second,
+                                                      (uint32_t)ilog2((uint32_t)plan->real_n) / 2, (hipStream_t)stream);
+        return e == hipSuccess ? TCFFT_SUCCESS : hip_fail(e);
+    }
     hipError_t e = launch_real(plan, -1, in, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
     return TCFFT_SUCCESS;
@@ -640,6 +665,15 @@ int tcfft_exec_c2r(tcfft_plan plan, const void* in, void* out, void* stream) {
     if (!in || !out || (a & 3u) || (b & 15u)) return TCFFT_INVALID_POINTER;
     if (a < b + half * batch * 4u && b < a + (half + 1) * batch * 4u) return TCFFT_INVALID_POINTER;  // overlap
     DeviceGuard guard(plan->device);
+    if (plan->real_ws) {
+        // n > 65536: the split into the workspace, then the inverse n/2-point transform
+        const hipError_t e = tcfft::launch_real_pairs(+1, (const uint32_t*)in, plan->real_ws,
+                                                      (uint32_t)ilog2((uint32_t)half), (uint32_t)batch,
+                                                      plan->w_real_pair.first, plan->w_real_pair.second,
+                                                      (uint32_t)ilog2((uint32_t)plan->real_n) / 2, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e);
+        return tcfft_exec_natural(plan, plan->real_ws, out, TCFFT_INVERSE, stream);
+    }
     hipError_t e = launch_real(plan, +1, in, out, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
     return TCFFT_SUCCESS;

@@ -13,7 +13,7 @@ backward of fft is the hand-written adjoint kernel (F^H) and the backward of
 fft_adjoint is the forward kernel (F).  No CPU fallback: these raise if the
 library or a GPU is missing.
 
-Real input (n = 512..65536):
+Real input (n = 2^9..2^27):
 
     rfft(x)          bins 0..n/2 of F x for real fp16 x [..., n] -> [..., n/2+1, 2]
     c2r(X, n)        y = sum over the Hermitian extension of X, n * irfft

@@ -155,3 +155,31 @@ def test_r2c_large_round_trip(n):
     note(f"[gpu] r2c round trip n={n}: rel_l2={err:.3e}")
     assert err <= 1.5e-3
     plan.destroy()
+
+
+@pytest.mark.parametrize("n,batch", [(1 << 17, 1), (1 << 17, 3), (1 << 18, 2), (1 << 20, 1)])
+def test_real_above_65536_matches_oracle(n, batch):
+    """Lengths above 65536: the n/2-point transform as the plan's passes plus
+    the separate even/odd pass, both directions and a round trip."""
+    note(f"[gpu] r2c/c2r two-step n={n} batch={batch}")
+    rng = np.random.default_rng(n + 7 * batch)
+    plan = tcfft_amd.PlanR2C(n, batch)
+    x = rng.uniform(-1, 1, (batch, n)).astype(np.float16).astype(np.float64)
+    X = r2c(plan, x)
+    err = oracle.rel_l2(X, oracle.rfft(x))
+    note(f"  r2c rel_l2={err:.3e}")
+    assert err <= 1.5e-3
+    Y = (rng.uniform(-1, 1, (batch, n // 2 + 1)) + 1j * rng.uniform(-1, 1, (batch, n // 2 + 1)))
+    Y = Y.real.astype(np.float16).astype(np.float64) + 1j * Y.imag.astype(np.float16).astype(np.float64)
+    y = c2r(plan, Y, n)
+    want = oracle.irfft(Y, n)
+    err = float(np.linalg.norm(y - want) / np.linalg.norm(want))
+    note(f"  c2r rel_l2={err:.3e}")
+    assert err <= 1.5e-3
+    # round trip on a scaled signal (n x must stay inside fp16's range)
+    xs = (x / 64).astype(np.float16).astype(np.float64) if n > (1 << 16) else x
+    back = c2r(plan, r2c(plan, xs), n) / n
+    err = float(np.linalg.norm(back - xs) / np.linalg.norm(xs))
+    note(f"  round trip rel_l2={err:.3e}")
+    assert err <= 2e-3
+    plan.destroy()
