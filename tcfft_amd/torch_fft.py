@@ -216,7 +216,8 @@ class _FusedConv(torch.autograd.Function):
         n, C = x.shape[-2], weight.shape[0]
         signals = x.numel() // (2 * n)
         rev, irev = _rev(n, x.device)
-        h_rev = weight.detach().index_select(1, rev).half().contiguous()
+        # cast, then permute (bit-identical to the other order, half the bytes gathered)
+        h_rev = weight.detach().half().index_select(1, rev).contiguous()
         plan = _plan(n, signals, x.device)
         stream = torch.cuda.current_stream(x.device).cuda_stream
         y = torch.empty_like(x)
