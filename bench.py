@@ -510,16 +510,18 @@ def bench_r2c(torch, tcfft_amd, reps):
             "ms_complex_of_real_data": round(ms_c, 5), "speedup_vs_complex": round(ms_c / ms_f, 3)}
 
 
-def hipfft_fp16(torch, dims, batch, reps):
+def hipfft_fp16(torch, dims, batch, reps, fp32=False):
     """The vendor library on the same workload: hipFFT (rocFFT) half-precision
     C2C through hipfftXtMakePlanMany(HIP_C_16F), in place, natural order,
-    timed with events over `reps` eager execs after a warm-up."""
+    timed with events over `reps` eager execs after a warm-up.  fp32: the
+    single-precision C2C (HIP_C_32F) instead, the vendor counterpart of the
+    split-precision path."""
     import ctypes
     try:
         lib = ctypes.CDLL("/opt/rocm/lib/libhipfft.so")
     except OSError as e:
         return {"error": f"libhipfft not loadable: {e}"}
-    HIP_C_16F = 6
+    ctype = 4 if fp32 else 6  # HIP_C_32F / HIP_C_16F
     h = ctypes.c_void_p()
     if lib.hipfftCreate(ctypes.byref(h)) != 0:
         return {"error": "hipfftCreate failed"}
@@ -527,9 +529,9 @@ def hipfft_fp16(torch, dims, batch, reps):
     n = (ctypes.c_longlong * rank)(*dims)
     total = int(np.prod(dims))
     ws = ctypes.c_size_t()
-    rc = lib.hipfftXtMakePlanMany(h, rank, n, None, ctypes.c_longlong(1), ctypes.c_longlong(total), HIP_C_16F, None,
-                                  ctypes.c_longlong(1), ctypes.c_longlong(total), HIP_C_16F, ctypes.c_longlong(batch),
-                                  ctypes.byref(ws), HIP_C_16F)
+    rc = lib.hipfftXtMakePlanMany(h, rank, n, None, ctypes.c_longlong(1), ctypes.c_longlong(total), ctype, None,
+                                  ctypes.c_longlong(1), ctypes.c_longlong(total), ctype, ctypes.c_longlong(batch),
+                                  ctypes.byref(ws), ctype)
     if rc != 0:
         lib.hipfftDestroy(h)
         return {"error": f"hipfftXtMakePlanMany returned {rc}"}
@@ -537,7 +539,9 @@ def hipfft_fp16(torch, dims, batch, reps):
     lib.hipfftSetStream(h, ctypes.c_void_p(stream))
     rng = np.random.default_rng(2)
     x = fp16_uniform(rng, (batch,) + tuple(dims))
-    bufs = rotation(torch, x.shape, torch.float16, torch.from_numpy(x))
+    if fp32:
+        x = x.astype(np.float32)
+    bufs = rotation(torch, x.shape, torch.float32 if fp32 else torch.float16, torch.from_numpy(x))
     lib.hipfftXtExec(h, ctypes.c_void_p(bufs[0].data_ptr()), ctypes.c_void_p(bufs[0].data_ptr()), -1)
     torch.cuda.synchronize()
     k = min(batch, 2)
@@ -548,8 +552,9 @@ def hipfft_fp16(torch, dims, batch, reps):
     ms = eager_time(torch, lambda i: lib.hipfftXtExec(h, ctypes.c_void_p(bufs[i % nb].data_ptr()),
                                                        ctypes.c_void_p(bufs[i % nb].data_ptr()), -1), reps)
     lib.hipfftDestroy(h)
-    out = summary(ms, total, batch, 8.0 * total * batch)
-    out.update({"rel_l2_vs_fp64": err, "library": "hipFFT/rocFFT (/opt/rocm), HIP_C_16F, in place, eager",
+    out = summary(ms, total, batch, (16.0 if fp32 else 8.0) * total * batch)
+    out.update({"rel_l2_vs_fp64": err,
+                "library": f"hipFFT/rocFFT (/opt/rocm), {'HIP_C_32F' if fp32 else 'HIP_C_16F'}, in place, eager",
                 "work_bytes": int(ws.value)})
     return out
 
@@ -721,6 +726,8 @@ def main():
         cfg["hipfft_fp16_n2^16_b256"] = hipfft_fp16(torch, [n], batch, reps)
         cfg["hipfft_fp16_n4096_b1024"] = hipfft_fp16(torch, [4096], 1024, reps)
         cfg["hipfft_fp16_2d_1024x1024_b8"] = hipfft_fp16(torch, [1024, 1024], 8, reps)
+        cfg["hipfft_fp16_n2^20_b16"] = hipfft_fp16(torch, [1 << 20], 16, reps)
+        cfg["hipfft_fp32_n2^16_b256"] = hipfft_fp16(torch, [n], batch, reps, fp32=True)
         cfg["hipfft_r2c_fp16_n8192_16384_signals"] = hipfft_r2c_fp16(torch, 8192, 64 * 256, reps)
         cfg["stream_copy_tbs"] = stream_copy(torch)
         out["configs"] = cfg
