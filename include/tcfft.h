@@ -175,8 +175,16 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * instead of all workgroups loading, computing and storing in lockstep.
  * TCFFT_OPT_COLS_DELAY (0..64, default 1): the same start delay (3 groups)
  * for a strided pass of radix 512..2048.
+ * TCFFT_OPT_SPLIT_DELAY (0..64, default 2): the same start delay (3 groups)
+ * for the two passes of tcfft_exec_split.
  * The schedule never changes results. */
-enum tcfft_option { TCFFT_OPT_SPLIT_CHUNK = 1, TCFFT_OPT_BIG_DELAY = 2, TCFFT_OPT_BIG_GROUPS = 3, TCFFT_OPT_COLS_DELAY = 4 };
+enum tcfft_option {
+    TCFFT_OPT_SPLIT_CHUNK = 1,
+    TCFFT_OPT_BIG_DELAY = 2,
+    TCFFT_OPT_BIG_GROUPS = 3,
+    TCFFT_OPT_COLS_DELAY = 4,
+    TCFFT_OPT_SPLIT_DELAY = 5
+};
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
 /* Times the plan's forward transform under each start schedule candidate

@@ -48,6 +48,7 @@ OPT_SPLIT_CHUNK = 1
 OPT_BIG_DELAY = 2
 OPT_BIG_GROUPS = 3
 OPT_COLS_DELAY = 4
+OPT_SPLIT_DELAY = 5
 
 
 class TcfftError(RuntimeError):

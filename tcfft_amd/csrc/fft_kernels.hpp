@@ -47,9 +47,11 @@ hipError_t launch_cols_big(int W, uint32_t* buf, uint32_t s_log2, uint32_t outer
                            StartSched sched, hipStream_t st);
 
 // Split-precision N = 65536 transform of FP32 complex data (in != out), as
-// two passes per chunk of `chunk` signals (0 = all at once).
+// two passes per chunk of `chunk` signals (0 = all at once); `sched` = the
+// passes' start schedule (delay and groups; the resident count is filled in).
 hipError_t launch_split_65536(const float2* in, float2* out, uint32_t n_signals, uint32_t chunk, int sign,
-                              const float2* w4096, const float2* wn_hi, const float2* wn_lo, hipStream_t st);
+                              const float2* w4096, const float2* wn_hi, const float2* wn_lo, StartSched sched,
+                              hipStream_t st);
 
 // Uneven batch: one task = signals [first, first + count) of one size class.
 struct MultiTask {

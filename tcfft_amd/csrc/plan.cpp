@@ -205,6 +205,7 @@ struct tcfft_plan_s {
     int big_delay = -1;   // start schedule of the first-pass kernels: -1 = default
     int big_groups = 3;
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
+    int split_delay = 2;  // start delay of the split-precision passes (3 groups; 2^16 x 256: 110.6 -> 106.6 us)
     int real_n = 0;       // real-input plan: the real length (its complex layout has n = real_n / 2)
     float2* w_real = nullptr;  // W_{real_n}^0..64 (n <= 8192) or ^0..64W (n = 8192 W <= 65536)
     std::pair<float2*, float2*> w_real_pair{};  // n > 65536: two-level table of W_n
@@ -691,6 +692,7 @@ int tcfft_exec_split(tcfft_plan plan, const void* in, void* out, int direction, 
     if (rc) return rc;
     hipError_t e = tcfft::launch_split_65536((const float2*)in, (float2*)out, (uint32_t)L.batch,
                                              (uint32_t)plan->split_chunk, direction, plan->w4096, t.first, t.second,
+                                             tcfft::StartSched{(uint32_t)plan->split_delay, 3u, 0u},
                                              (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
     return TCFFT_SUCCESS;
@@ -795,6 +797,10 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
     case TCFFT_OPT_BIG_GROUPS:
         if (value < 1 || value > 8) return TCFFT_INVALID_VALUE;
         plan->big_groups = value;
+        return TCFFT_SUCCESS;
+    case TCFFT_OPT_SPLIT_DELAY:
+        if (value < 0 || value > 64) return TCFFT_INVALID_VALUE;
+        plan->split_delay = value;
         return TCFFT_SUCCESS;
     default: return TCFFT_INVALID_VALUE;
     }

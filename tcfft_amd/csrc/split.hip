@@ -98,8 +98,15 @@ __device__ __forceinline__ void wave_scale(float m, float& s, float& inv) {
 template <int PASS, int SIGN>
 __global__ __launch_bounds__(256, 2) void k_split_col(const float2* __restrict__ src, float2* __restrict__ dst,
                                                    const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
-                                                   const float2* __restrict__ wn_lo) {
+                                                   const float2* __restrict__ wn_lo, StartSched sched) {
     constexpr float sg = SIGN < 0 ? 1.f : -1.f;
+    // start schedule (as the first-pass kernels): the resident workgroups in
+    // `groups` groups by their index within an XCD, group k sleeping k * delay
+    // quanta of 1024 cycles, so that loads, compute and stores overlap
+    if (sched.delay && blockIdx.x < sched.resident) {
+        const u32 quanta = ((blockIdx.x >> 3) % sched.groups) * sched.delay;
+        for (u32 i = 0; i < quanta; ++i) __builtin_amdgcn_s_sleep(16);
+    }
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const int c16 = lane & 15, g = lane >> 4;
     const u32 sig = blockIdx.x >> 2;
@@ -244,25 +251,44 @@ __global__ __launch_bounds__(256, 2) void k_split_col(const float2* __restrict__
     }
 }
 
+// Workgroups of 256 threads of kernel k resident at once on the current device.
+template <class K>
+static u32 split_resident(K k) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, 256, 0) != hipSuccess)
+        return 0;  // no schedule rather than a wrong one
+    return (u32)(cus * per_cu);
+}
+
 template <int SIGN>
 static hipError_t launch_split_t(const float2* in, float2* out, u32 n_signals, u32 chunk, const float2* w4096,
-                                 const float2* wn_hi, const float2* wn_lo, hipStream_t st) {
+                                 const float2* wn_hi, const float2* wn_lo, StartSched sched, hipStream_t st) {
+    // one cached count per kernel (each instantiation of this function and
+    // each variable below is a distinct static)
+    static const u32 res1 = split_resident(k_split_col<1, SIGN>);
+    static const u32 res2 = split_resident(k_split_col<2, SIGN>);
+    if (sched.groups == 0) sched.delay = 0;
     if (chunk == 0 || chunk > n_signals) chunk = n_signals;
     for (u32 s0 = 0; s0 < n_signals; s0 += chunk) {
         const u32 cnt = min(chunk, n_signals - s0);
         const size_t off = (size_t)s0 * 65536;
+        StartSched s1 = sched, s2 = sched;
+        s1.resident = res1;
+        s2.resident = res2;
         hipLaunchKernelGGL((k_split_col<1, SIGN>), dim3(cnt * 4), dim3(256), 0, st, in + off, out + off, w4096, wn_hi,
-                           wn_lo);
+                           wn_lo, s1);
         hipLaunchKernelGGL((k_split_col<2, SIGN>), dim3(cnt * 4), dim3(256), 0, st, out + off, out + off, w4096, wn_hi,
-                           wn_lo);
+                           wn_lo, s2);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_split_65536(const float2* in, float2* out, u32 n_signals, u32 chunk, int sign, const float2* w4096,
-                              const float2* wn_hi, const float2* wn_lo, hipStream_t st) {
-    return sign < 0 ? launch_split_t<-1>(in, out, n_signals, chunk, w4096, wn_hi, wn_lo, st)
-                    : launch_split_t<+1>(in, out, n_signals, chunk, w4096, wn_hi, wn_lo, st);
+                              const float2* wn_hi, const float2* wn_lo, StartSched sched, hipStream_t st) {
+    return sign < 0 ? launch_split_t<-1>(in, out, n_signals, chunk, w4096, wn_hi, wn_lo, sched, st)
+                    : launch_split_t<+1>(in, out, n_signals, chunk, w4096, wn_hi, wn_lo, sched, st);
 }
 
 }  // namespace tcfft

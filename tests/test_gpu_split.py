@@ -80,3 +80,25 @@ def test_split_rejects_bad_arguments():
     with pytest.raises(tcfft_amd.TcfftError):
         p2.exec_split(t.data_ptr(), u.data_ptr())  # size not supported
     p2.destroy()
+
+
+def test_split_start_delay_bit_identical():
+    """The split passes' start schedule (TCFFT_OPT_SPLIT_DELAY) never changes
+    results; out-of-range values are rejected."""
+    rng = np.random.default_rng(41)
+    n, batch = 1 << 16, 24
+    x = torch.from_numpy(rng.standard_normal((batch, n, 2)).astype(np.float32)).cuda()
+    plan = tcfft_amd.Plan1D(n, batch)
+    plan.set_option(tcfft_amd.OPT_SPLIT_DELAY, 0)
+    ref = torch.empty_like(x)
+    plan.exec_split(x.data_ptr(), ref.data_ptr())
+    for d in (1, 2, 5, 64):
+        plan.set_option(tcfft_amd.OPT_SPLIT_DELAY, d)
+        out = torch.empty_like(x)
+        plan.exec_split(x.data_ptr(), out.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), d
+    for bad in (-1, 65):
+        with pytest.raises(tcfft_amd.TcfftError):
+            plan.set_option(tcfft_amd.OPT_SPLIT_DELAY, bad)
+    plan.destroy()
