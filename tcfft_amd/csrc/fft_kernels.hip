@@ -254,7 +254,7 @@ __device__ __forceinline__ void nat_rows_load(u32 (&d)[16][4], const u32* __rest
 template <int T, int SIGN>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* __restrict__ in, u32* __restrict__ out,
                                                                   u32 n_tiles, const float2* __restrict__ w4096,
-                                                                  u32 nx_log2, RevSpec x_pos) {
+                                                                  u32 nx_log2, RevSpec x_pos, StartSched sched) {
     extern __shared__ u32 lds[];
     constexpr u32 L = 256u * T;
     const int lane = threadIdx.x & 63;
@@ -262,6 +262,7 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* _
     const u32 wave = blockIdx.x * (blockDim.x >> 6) + wib;
     const u32 tile0 = wave * 16u;
     if (tile0 >= n_tiles) return;
+    start_delay(sched);
     const u32 ntl = min(16u, n_tiles - tile0);
     u32* const wl = lds + wib * 4096u;
     const int c16 = lane & 15, g = lane >> 4;
@@ -1200,29 +1201,31 @@ static hipError_t launch_cols_t(u32* buf, u32 s_log2, u32 outer, int sign, const
 
 template <int T, int SIGN>
 static hipError_t launch_rows_nat_t(const u32* in, u32* out, u32 n_tiles, const float2* w4096, u32 nx_log2,
-                                    const RevSpec& x_pos, hipStream_t st) {
+                                    const RevSpec& x_pos, StartSched sched, hipStream_t st) {
     constexpr size_t lds = 4 * 4096 * sizeof(u32);  // 16 KiB per wave
     auto k = k_rows_nat<T, SIGN>;
     const KernelInfo ki = kernel_info(k, 256, lds);
     if (ki.lds_status != hipSuccess) return ki.lds_status;
+    if (sched.groups == 0) sched.delay = 0;
+    if (sched.delay) sched.resident = ki.resident;
     const u32 waves = (n_tiles + 15) / 16;
-    hipLaunchKernelGGL(k, dim3((waves + 3) / 4), dim3(256), lds, st, in, out, n_tiles, w4096, nx_log2, x_pos);
+    hipLaunchKernelGGL(k, dim3((waves + 3) / 4), dim3(256), lds, st, in, out, n_tiles, w4096, nx_log2, x_pos, sched);
     return hipGetLastError();
 }
 
 hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign, const float2* w4096, u32 nx_log2,
-                           const RevSpec& x_pos, hipStream_t st) {
+                           const RevSpec& x_pos, StartSched sched, hipStream_t st) {
     switch (T * 2 + (sign < 0 ? 0 : 1)) {
-    case 2: return launch_rows_nat_t<1, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 3: return launch_rows_nat_t<1, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 4: return launch_rows_nat_t<2, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 5: return launch_rows_nat_t<2, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 8: return launch_rows_nat_t<4, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 9: return launch_rows_nat_t<4, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 16: return launch_rows_nat_t<8, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 17: return launch_rows_nat_t<8, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 32: return launch_rows_nat_t<16, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
-    case 33: return launch_rows_nat_t<16, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, st);
+    case 2: return launch_rows_nat_t<1, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 3: return launch_rows_nat_t<1, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 4: return launch_rows_nat_t<2, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 5: return launch_rows_nat_t<2, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 8: return launch_rows_nat_t<4, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 9: return launch_rows_nat_t<4, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 16: return launch_rows_nat_t<8, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 17: return launch_rows_nat_t<8, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 32: return launch_rows_nat_t<16, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 33: return launch_rows_nat_t<16, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
     default: return hipErrorInvalidValue;
     }
 }

@@ -959,7 +959,7 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
             x_pos = rev_spec(rx);
         }
         hipError_t e = fused_rows ? tcfft::launch_rows_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
-                                                           plan->w4096, nx_log2, x_pos, st)
+                                                           plan->w4096, nx_log2, x_pos, rows_sched(plan, first), st)
                                   : tcfft::launch_big_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
                                                           plan->w4096, first.big_hi, first.big_lo,
                                                           first.big_M_log2 / 2, nx_log2, x_pos,

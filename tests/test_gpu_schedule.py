@@ -153,3 +153,22 @@ def test_radix_split_parity(n, batch):
     assert rel_l2(t, want) <= tol
     plan.destroy()
 
+
+
+@pytest.mark.parametrize("n,batch", [(4096, 1024), (1024, 300)])
+def test_natural_rows_schedule_bit_identical(n, batch):
+    """The start schedule of the natural-order rows pass (TCFFT_OPT_BIG_DELAY)
+    never changes results."""
+    rng = np.random.default_rng(n)
+    x = torch.from_numpy(rng.uniform(-1, 1, (batch, n, 2)).astype(np.float16)).cuda()
+    plan = tcfft_amd.Plan1D(n, batch)
+    plan.set_option(tcfft_amd.OPT_BIG_DELAY, 0)
+    ref = torch.empty_like(x)
+    plan.exec_natural(x.data_ptr(), ref.data_ptr())
+    for d in (-1, 1, 3):
+        plan.set_option(tcfft_amd.OPT_BIG_DELAY, d)
+        out = torch.empty_like(x)
+        plan.exec_natural(x.data_ptr(), out.data_ptr())
+        torch.cuda.synchronize()
+        assert torch.equal(out, ref), d
+    plan.destroy()
