@@ -216,7 +216,7 @@ __device__ __forceinline__ void stepb_read(u32 (&d)[16][4], const u32* lds, int 
 // k_dif: natural-order x (N = 4096*W per signal) -> spectrum in the drop-in's
 // digit-reversed order, out of place (in != out) or in place.
 template <int W, int SIGN>
-__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_dif(const u32* __restrict__ in, u32* __restrict__ out,
+__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_dif(const u32* in, u32* out,
                                                 const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
                                                 const float2* __restrict__ wn_lo, u32 wn_lo_bits) {
     extern __shared__ u32 lds[];
@@ -247,9 +247,11 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_dif(const u32* __r
 // The product is taken on the FP32 results of the last DIF stage; the
 // inverse is the DIT of the single-pass kernel (sign +1) and the scale is
 // applied to its FP32 results.
+// x and y may alias (in place), so neither is __restrict__: a workgroup reads
+// its signal completely before its first store of y.
 template <int W, bool CONJ, bool SPEC>
-__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_conv(const u32* __restrict__ x, const u32* __restrict__ h_rev,
-                                                 u32* __restrict__ y, u32* __restrict__ spec, u32 channels,
+__global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_conv(const u32* x, const u32* __restrict__ h_rev,
+                                                 u32* y, u32* __restrict__ spec, u32 channels,
                                                  float scale, const float2* __restrict__ w4096,
                                                  const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo,
                                                  u32 wn_lo_bits) {

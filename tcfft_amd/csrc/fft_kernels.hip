@@ -252,7 +252,7 @@ __device__ __forceinline__ void nat_rows_load(u32 (&d)[16][4], const u32* __rest
 }
 
 template <int T, int SIGN>
-__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* __restrict__ in, u32* __restrict__ out,
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* in, u32* out,
                                                                   u32 n_tiles, const float2* __restrict__ w4096,
                                                                   u32 nx_log2, RevSpec x_pos, StartSched sched) {
     extern __shared__ u32 lds[];
@@ -859,7 +859,7 @@ __device__ __forceinline__ u32 nat16_word(u32 n) {
 
 template <int W, int SIGN>
 __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
-    const u32* __restrict__ in, u32* __restrict__ out, const float2* __restrict__ w4096,
+    const u32* in, u32* out, const float2* __restrict__ w4096,
     const float2* __restrict__ wn_hi, const float2* __restrict__ wn_lo, u32 wn_lo_bits, u32 nx_log2, RevSpec x_pos,
     StartSched sched) {
     extern __shared__ u32 lds[];

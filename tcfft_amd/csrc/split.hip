@@ -95,8 +95,11 @@ __device__ __forceinline__ void wave_scale(float m, float& s, float& inv) {
     inv = __uint_as_float((u32)ib << 23);
 }
 
+// src and dst alias in pass 2 (in place on the intermediate), so neither is
+// __restrict__: a workgroup reads its 64 columns completely before its first
+// store, and workgroups own disjoint columns.
 template <int PASS, int SIGN>
-__global__ __launch_bounds__(256, 2) void k_split_col(const float2* __restrict__ src, float2* __restrict__ dst,
+__global__ __launch_bounds__(256, 2) void k_split_col(const float2* src, float2* dst,
                                                    const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
                                                    const float2* __restrict__ wn_lo, StartSched sched) {
     constexpr float sg = SIGN < 0 ? 1.f : -1.f;
