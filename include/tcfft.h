@@ -104,6 +104,14 @@ int tcfft_exec(tcfft_plan plan, void *data, int direction, void *stream);
  * captured into a hipGraph. */
 int tcfft_exec_natural(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
 
+/* The caller-side staging of tcfftExec on the device: for a 1D plan of n
+ * points, out[b*n + p] = in[b*n + rev[p]] for n_lines lines (rev = the
+ * plan's gen_rev order, tcfft_doit_half.cpp:6-29,40-46), as one LDS-tiled
+ * pass at copy speed.  4-byte elements (fp16 complex), out of place (no
+ * overlap), 16-byte aligned; n_lines is independent of the plan's batch.
+ * Allocates nothing; graph-capturable. */
+int tcfft_permute_to_dropin(tcfft_plan plan, const void *in, void *out, int n_lines, void *stream);
+
 /* Split precision (no counterpart in the reference; config "FP16 hi+lo of
  * FP32 inputs"): FP32 complex input and output (interleaved float re/im,
  * natural order, in != out), every radix-16 stage as FP16 hi/lo products on

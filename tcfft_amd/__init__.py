@@ -42,7 +42,7 @@ EXPORTS = (
     "tcfft_multi_set_schedule", "tcfft_multi_autotune", "tcfft_multi_set_start_delay", "tcfft_multi_get_schedule",
     "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad", "tcfft_plan_autotune",
     "tcfft_plan_create_1d_split", "tcfft_plan_create_1d_tuned", "tcfft_query_1d_split",
-    "tcfft_plan_create_r2c_1d", "tcfft_exec_r2c", "tcfft_exec_c2r",
+    "tcfft_plan_create_r2c_1d", "tcfft_exec_r2c", "tcfft_exec_c2r", "tcfft_permute_to_dropin",
 )
 OPT_SPLIT_CHUNK = 1
 OPT_BIG_DELAY = 2
@@ -97,6 +97,7 @@ def lib():
         L.tcfft_query_1d_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int, _i32p, _i32p]
         L.tcfft_plan_create_r2c_1d.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int]
         L.tcfft_exec_r2c.argtypes = [_vp, _vp, _vp, _vp]
+        L.tcfft_permute_to_dropin.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_exec_c2r.argtypes = [_vp, _vp, _vp, _vp]
         L.tcfft_exec_dif.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_conv_exec.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp, ctypes.c_int, _vp, _vp]
@@ -194,6 +195,12 @@ class _Plan:
     def exec_natural(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
         _check(lib().tcfft_exec_natural(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)),
                "tcfft_exec_natural")
+
+    def permute_to_dropin(self, in_ptr: int, out_ptr: int, n_lines: int, stream: int = 0):
+        """out[b, p] = in[b, rev[p]] for n_lines lines of the plan's n (1D):
+        natural-order data into the order exec consumes, on the device."""
+        _check(lib().tcfft_permute_to_dropin(self._h, _vp(in_ptr), _vp(out_ptr), n_lines, _vp(stream)),
+               "tcfft_permute_to_dropin")
 
     def exec_split(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
         """FP32 complex in/out (natural order, in != out), FP16 hi/lo MFMA split precision."""

@@ -985,6 +985,26 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
     return run_passes(plan, (uint32_t*)out, direction, st);
 }
 
+int tcfft_permute_to_dropin(tcfft_plan plan, const void* in, void* out, int n_lines, void* stream) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    const Layout& L = plan->lay;
+    if (L.dims != 1) return TCFFT_INVALID_PLAN;
+    if (n_lines < 1 || (long)L.n * n_lines >= (1L << 31)) return TCFFT_INVALID_BATCH;
+    const uintptr_t a = (uintptr_t)in, b = (uintptr_t)out;
+    const size_t bytes = (size_t)L.n * (size_t)n_lines * 4u;
+    if (!in || !out || (a & 15u) || (b & 15u)) return TCFFT_INVALID_POINTER;
+    if (a < b + bytes && b < a + bytes) return TCFFT_INVALID_POINTER;  // overlap
+    tcfft::GatherTile gt{};
+    if (!gather_tile(L.rad_y, gt)) return TCFFT_INVALID_VALUE;
+    DeviceGuard guard(plan->device);
+    const hipError_t e = tcfft::launch_gather_tiled_1d((const uint32_t*)in, (uint32_t*)out,
+                                                       (uint32_t)ilog2((uint32_t)L.n),
+                                                       (uint32_t)((size_t)L.n * (size_t)n_lines),
+                                                       rev_spec(L.rad_y), gt, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
 int tcfft_gen_rev(int n, const int* radices, int n_radices, int* rev) {
     if (!radices || !rev) return TCFFT_INVALID_POINTER;
     if (n_radices < 0 || n_radices > 64) return TCFFT_INVALID_VALUE;

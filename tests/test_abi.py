@@ -165,3 +165,8 @@ def test_real_input_entry_points_validate_without_device():
     assert h.value is None
     assert L.tcfft_exec_r2c(None, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), None) == 4
     assert L.tcfft_exec_c2r(None, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), None) == 4
+
+
+def test_permute_to_dropin_validates_without_device():
+    L = tcfft_amd.lib()
+    assert L.tcfft_permute_to_dropin(None, ctypes.c_void_p(0x1000), ctypes.c_void_p(0x2000), 1, None) == 4
