@@ -1438,8 +1438,10 @@ template <int T, int SIGN>
 __device__ __forceinline__ void multi_rows(u32* buf, const MultiTask& tk, const u32* __restrict__ offsets,
                                            const float2* w4096) {
     constexpr int LPW = 16 / T;  // lines per wave
-    // wave-uniform (scalar) so that the per-line offsets are scalar loads
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // wave-uniform (scalar) so that the per-line offsets are scalar loads;
+    // lane_id() keeps the lane offset inside the task loop (hoisted out of it,
+    // it stays live across every size class and the kernel spills)
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = lane_id();
     const int first_line = wave * LPW;
     const int lines = min((int)tk.count - first_line, LPW);
     if (lines <= 0) return;
