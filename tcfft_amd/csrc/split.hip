@@ -186,10 +186,13 @@ __global__ __launch_bounds__(256, 2) void k_split_col(const float2* src, float2*
         // X[k + 256 m], m = k0 + 16 k1: column col, row m; each tile is stored
         // as soon as it is transformed
         float2* q = dst + sbase + col + 16384u * g;
+        // the DFT-16 operand again, from the opaque lane id: kept live from
+        // stage 1 instead, it is spilled (12 VGPRs of scratch per lane)
+        const Dft16Split F2 = make_dft16_split(lane_id(), SIGN);
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             f32x4 ore, oim;
-            dft16_split_b<false>(dr[t], di[t], 1.f, F, ore, oim);
+            dft16_split_b<false>(dr[t], di[t], 1.f, F2, ore, oim);
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 const f32x2 v = {ore[i] * inv2, oim[i] * inv2};
