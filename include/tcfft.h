@@ -224,7 +224,8 @@ int tcfft_multi_set_schedule(tcfft_multi plan, int grid, int order);
 int tcfft_multi_set_start_delay(tcfft_multi plan, int delay);
 int tcfft_multi_get_schedule(tcfft_multi plan, int *grid, int *order, int *delay);
 /* Times every (grid, order, start delay 0..2) candidate on `data` (which it
- * overwrites) and keeps the fastest. */
+ * overwrites), times the four fastest again in three alternating rounds,
+ * and keeps the one with the lowest time. */
 int tcfft_multi_autotune(tcfft_multi plan, void *data, void *stream, int *best_grid, int *best_order,
                          float *best_ms);
 

@@ -1252,31 +1252,59 @@ int tcfft_multi_autotune(tcfft_multi plan, void* data, void* stream, int* best_g
     int cu = 256;
     (void)hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, plan->device);
     const int grids[] = {cu / 2, cu, 2 * cu};
-    float best = 1e30f;
-    int bg = cu, bo = 0, rc = TCFFT_SUCCESS;
-    int bd = 0;
-    // every (task order, grid, start delay), two warm-up runs and five timed
+    struct Cand {
+        int grid, order, delay;
+        float ms;
+    };
+    std::vector<Cand> cands;
+    int rc = TCFFT_SUCCESS;
+    // mean of five executions after two warm-up ones, on the current schedule
+    auto time_current = [&](float& ms) {
+        for (int i = 0; i < 2 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
+        if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+        for (int i = 0; i < 5 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
+        if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+        if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+        ms = 0.f;
+        if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+        ms /= 5.f;
+    };
+    auto apply = [&](const Cand& c) {
+        if (!rc) rc = tcfft_multi_set_schedule(plan, c.grid, c.order);
+        plan->delay = c.delay;
+    };
+    // every (task order, grid, start delay) once
     for (int order = 0; order < 3 && !rc; ++order)
         for (int gsz : grids)
             for (int delay = 0; delay <= 2 && !rc; ++delay) {
-                rc = tcfft_multi_set_schedule(plan, gsz, order);
-                if (rc) break;
-                plan->delay = delay;
-                for (int i = 0; i < 2 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
-                if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
-                for (int i = 0; i < 5 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
-                if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
-                if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
-                float ms = 0.f;
-                if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
-                ms /= 5.f;
-                if (!rc && ms < best) {
-                    best = ms;
-                    bg = gsz;
-                    bo = order;
-                    bd = delay;
-                }
+                Cand c{gsz, order, delay, 0.f};
+                apply(c);
+                time_current(c.ms);
+                if (!rc) cands.push_back(c);
             }
+    // the four fastest again, three rounds in turn, each keeping its best
+    // round: one sweep's timings differ by several percent between
+    // neighbouring candidates, enough to pick a slower schedule
+    if (!rc) {
+        std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+        if (cands.size() > 4) cands.resize(4);
+        for (int round = 0; round < 3 && !rc; ++round)
+            for (Cand& c : cands) {
+                float ms = 0.f;
+                apply(c);
+                time_current(ms);
+                if (!rc) c.ms = std::min(c.ms, ms);
+            }
+    }
+    float best = 1e30f;
+    int bg = cu, bo = 0, bd = 0;
+    for (const Cand& c : cands)
+        if (c.ms < best) {
+            best = c.ms;
+            bg = c.grid;
+            bo = c.order;
+            bd = c.delay;
+        }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc) return rc;
