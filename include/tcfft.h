@@ -225,7 +225,9 @@ int tcfft_multi_set_start_delay(tcfft_multi plan, int delay);
 int tcfft_multi_get_schedule(tcfft_multi plan, int *grid, int *order, int *delay);
 /* Times every (grid, order, start delay 0..2) candidate on `data` (which it
  * overwrites), times the four fastest again in three alternating rounds,
- * and keeps the one with the lowest time. */
+ * and keeps the one with the lowest time.  Each timed run follows a write of
+ * a 256 MiB buffer that the call allocates and frees (skipped if the
+ * allocation fails), so that the data is timed cold, not cache-resident. */
 int tcfft_multi_autotune(tcfft_multi plan, void *data, void *stream, int *best_grid, int *best_order,
                          float *best_ms);
 

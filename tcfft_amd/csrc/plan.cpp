@@ -1258,16 +1258,32 @@ int tcfft_multi_autotune(tcfft_multi plan, void* data, void* stream, int* best_g
     };
     std::vector<Cand> cands;
     int rc = TCFFT_SUCCESS;
-    // mean of five executions after two warm-up ones, on the current schedule
+    // Back to back on one buffer, a run finds part of the data still in the
+    // 256 MB last-level cache, which favours some schedules over others.
+    // Before each timed run the tuner therefore writes a buffer of that size
+    // (when it can allocate one), as a caller's previous work would.
+    const size_t flush_bytes = size_t(256) << 20;
+    void* flush = nullptr;
+    if (hipMalloc(&flush, flush_bytes) != hipSuccess) {
+        flush = nullptr;
+        (void)hipGetLastError();
+    }
+    // mean of five single timed executions after two warm-up ones, on the
+    // current schedule
     auto time_current = [&](float& ms) {
         for (int i = 0; i < 2 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
-        if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
-        for (int i = 0; i < 5 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
-        if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
-        if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
-        ms = 0.f;
-        if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
-        ms /= 5.f;
+        float total = 0.f;
+        for (int i = 0; i < 5 && !rc; ++i) {
+            if (flush && hipMemsetAsync(flush, i + 1, flush_bytes, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            if (!rc) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
+            if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            float t = 0.f;
+            if (!rc && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            total += t;
+        }
+        ms = total / 5.f;
     };
     auto apply = [&](const Cand& c) {
         if (!rc) rc = tcfft_multi_set_schedule(plan, c.grid, c.order);
@@ -1305,6 +1321,7 @@ int tcfft_multi_autotune(tcfft_multi plan, void* data, void* stream, int* best_g
             bo = c.order;
             bd = c.delay;
         }
+    if (flush) (void)hipFree(flush);
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     if (rc) return rc;
