@@ -77,6 +77,9 @@ def test_newer_entry_points_reject_bad_arguments_without_device():
     assert L.tcfft_plan_set_option(None, 2, 1) == 4
     assert L.tcfft_multi_set_start_delay(None, 1) == 4
     assert L.tcfft_multi_get_schedule(None, None, None, None) == 4
+    assert L.tcfft_multi_set_schedule(None, 256, 0) == 4
+    assert L.tcfft_multi_exec(None, ctypes.c_void_p(0x1000), -1, None) == 4
+    assert L.tcfft_multi_autotune(None, ctypes.c_void_p(0x1000), None, None, None, None) == 4
     rad = (ctypes.c_int * 32)()
     nr, npass = ctypes.c_int(), ctypes.c_int()
     assert L.tcfft_query_1d_split(1 << 20, 1, 13, rad, 32, ctypes.byref(nr), ctypes.byref(npass)) == 5
