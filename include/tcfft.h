@@ -15,6 +15,8 @@
 #ifndef TCFFT_H
 #define TCFFT_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -98,10 +100,11 @@ int tcfft_exec(tcfft_plan plan, void *data, int direction, void *stream);
  * bit.  The digit reversal runs in the first pass's load for lines (1D, or
  * 2D rows) of up to 65536 points, else as one extra tiled reversal pass.  in == out is allowed; in and out must not otherwise
  * overlap.  The 2D in-place case and the 1D in-place case above 65536
- * points stage the input in a plan-owned device workspace of the buffer's size
- * (allocated on first use), so concurrent in-place calls on one plan from
- * different streams race on it; all other cases allocate nothing and can be
- * captured into a hipGraph. */
+ * points stage the input in a device workspace of the buffer's size that such
+ * plans allocate at creation (2D plans, 1D plans whose first pass does not
+ * read natural order), so concurrent in-place calls on one plan from
+ * different streams race on it.  No call allocates, frees or memsets: every
+ * case can be captured into a hipGraph. */
 int tcfft_exec_natural(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
 
 /* The caller-side staging of tcfftExec on the device: for a 1D plan of n
@@ -196,9 +199,14 @@ enum tcfft_option {
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
 /* Times the plan's forward transform under each start schedule candidate
- * (groups, delay) on `data` (which it overwrites with transforms of
- * itself), keeps the fastest in the plan and reports it; then the column
- * pass delay (0..2) when the plan has a strided pass of radix 512..2048.  No reference
+ * of its first pass (no stagger, or 3, 5 or 7 groups with delays of 1..4
+ * quanta) on `data` (which it overwrites with transforms of itself), keeps
+ * the fastest in the plan and reports it; then the column pass delay (0..4)
+ * when the plan has a strided pass of radix 512..2048.  Every timed run
+ * follows a write of a 256 MiB buffer that the call allocates and frees
+ * (skipped if the allocation fails), so that the data is timed cold, as a
+ * caller's previous work leaves it; the four fastest candidates are timed
+ * again in three alternating rounds before the choice.  No reference
  * counterpart (the reference has one fixed launch per size). */
 int tcfft_plan_autotune(tcfft_plan plan, void *data, void *stream, int *best_groups, int *best_delay,
                         float *best_ms);

@@ -98,18 +98,18 @@ __device__ __forceinline__ void dit4096(u32 (&d)[16][4], int sign, const Dit4096
         step[i] = cf{tw.step[i].x, sg * tw.step[i].y};
         rec[i] = REC0 ? rec0[i] : step[i];
     }
-    // stage 0: DFT over d0 -> lane%16 = k0, (g,i) = d1
+    // Stages 0 and 1 only combine points of one tile, so both run tile by
+    // tile: tile t needs only its own load, and with the loads issued in tile
+    // order (they complete in issue order) its work starts as soon as that
+    // load has arrived instead of after the last one.
 #pragma unroll
     for (int t = 0; t < 16; ++t) {
+        // stage 0: DFT over d0 -> lane%16 = k0, (g,i) = d1
         f32x4 re, im;
         dft16_a(d[t], op, re, im);
 #pragma unroll
         for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
-    }
-    // stage 1: DFT over d1 -> lane%16 = k1, (g,i) = k0, tile = d2; twiddle W_4096^{c*d2}
-#pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
+        // stage 1: DFT over d1 -> lane%16 = k1, (g,i) = k0, tile = d2; twiddle W_4096^{c*d2}
         dft16_a(d[t], op, re, im);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
@@ -375,7 +375,10 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
     {
         const u32* src = sbase + 4096 * w + 16 * c16 + 4 * g;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) load4_nt(d[t], src + 256 * t);
+        for (int t = 0; t < 16; ++t) {
+            load4_nt(d[t], src + 256 * t);
+            __builtin_amdgcn_sched_barrier(0);  // keep the tile order of the loads
+        }
     }
     __builtin_amdgcn_sched_barrier(0);
     const Dft16Op op = make_dft16(lane, SIGN);

@@ -14,12 +14,14 @@
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
+#include <functional>
 #include <map>
 #include <new>
 #include <vector>
 
 #include "../../include/tcfft.h"
 #include "fft_kernels.hpp"
+#include "plan_internal.hpp"
 
 using tcfft::EpiParams;
 
@@ -197,6 +199,11 @@ struct tcfft_plan_s {
     float2* w4096 = nullptr;
     std::map<uint32_t, std::pair<float2*, float2*>> epi_tables;  // M_log2 -> (hi, lo)
     std::vector<void*> allocs;
+    // Staging copy of the input for in-place tcfft_exec_natural on layouts
+    // whose first pass cannot read its input in place (see
+    // natural_needs_stage); allocated with the plan so that no exec call
+    // allocates.  One per plan: in-place natural-order calls on one plan
+    // must not overlap.
     void* workspace = nullptr;
     size_t workspace_bytes = 0;
     size_t total = 0;  // elements in the buffer
@@ -244,9 +251,19 @@ int upload(tcfft_plan_s* p, const std::vector<float2>& h, float2** dst) {
 
 void free_plan(tcfft_plan_s* p) {
     for (void* a : p->allocs) (void)hipFree(a);
-    if (p->workspace) (void)hipFree(p->workspace);
     p->magic = 0;
     delete p;
+}
+
+// Whether in-place tcfft_exec_natural stages a copy of its input: 2D plans
+// (the first pass writes the x reversal as a permutation of whole lines, so a
+// workgroup would overwrite lines others have not read yet) and 1D plans
+// whose first pass does not read natural order itself (the reversal pass
+// before it gathers from the whole line).
+bool natural_needs_stage(const Layout& L) {
+    const Pass& f = L.passes[0];
+    const bool fused = (f.kind == 0 || f.kind == 2) && !f.has_epi;
+    return L.dims == 2 || !fused;
 }
 
 // Two-level table of W_M, M = 2^M_log2: hi[k] = W_M^{k*2^lo}, lo[k] = W_M^k,
@@ -291,12 +308,25 @@ int finish_plan(tcfft_plan_s* p) {
     return TCFFT_SUCCESS;
 }
 
-int create_common(tcfft_plan* plan, const Layout& lay) {
+// stage_ws: allocate the staging workspace of in-place natural order when
+// the layout needs one (real-input plans pass false and lend their own
+// workspace of the same size instead).
+int create_common(tcfft_plan* plan, const Layout& lay, bool stage_ws = true) {
     tcfft_plan_s* p = new (std::nothrow) tcfft_plan_s;
     if (!p) return TCFFT_ALLOC_FAILED;
     p->lay = lay;
     p->total = lay.dims == 1 ? (size_t)lay.n * lay.batch : (size_t)lay.nx * lay.ny * lay.batch;
     int rc = finish_plan(p);
+    if (!rc && stage_ws && natural_needs_stage(lay)) {
+        const hipError_t e = hipMalloc(&p->workspace, p->total * 4u);
+        if (e != hipSuccess) {
+            p->workspace = nullptr;
+            rc = hip_fail(e);
+        } else {
+            p->allocs.push_back(p->workspace);
+            p->workspace_bytes = p->total * 4u;
+        }
+    }
     if (rc) {
         free_plan(p);
         return rc;
@@ -608,7 +638,7 @@ int tcfft_plan_create_r2c_1d(tcfft_plan* plan, int n, int n_batch) {
     Layout lay;
     int rc = build_layout_1d(n / 2, n_batch, lay);
     if (rc) return rc;
-    rc = create_common(plan, lay);
+    rc = create_common(plan, lay, false);
     if (rc) return rc;
     tcfft_plan_s* p = *plan;
     p->real_n = n;
@@ -625,6 +655,10 @@ int tcfft_plan_create_r2c_1d(tcfft_plan* plan, int n, int n_batch) {
             else {
                 p->allocs.push_back(ws);
                 p->real_ws = (uint32_t*)ws;
+                // the half-size complex plan's in-place natural order stages
+                // through the same buffer (both hold n/2 * batch words)
+                p->workspace = ws;
+                p->workspace_bytes = (size_t)n / 2 * (size_t)n_batch * 4u;
             }
         }
     }
@@ -818,82 +852,93 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
         (void)hipEventDestroy(a);
         return TCFFT_HIP_ERROR;
     }
-    struct Cand {
-        int groups, delay;
-        float ms;
-    };
-    std::vector<Cand> cands;
-    for (int d = 0; d <= 3; ++d) cands.push_back({3, d, 1e30f});
-    cands.push_back({5, 1, 1e30f});
-    cands.push_back({5, 2, 1e30f});
-    cands.push_back({7, 1, 1e30f});
-    const int saved_groups = plan->big_groups, saved_delay = plan->big_delay;
-    const int reps = 10;
     int rc = TCFFT_SUCCESS;
-    // two interleaved rounds, the faster of each candidate's two timings kept
-    for (int round = 0; round < 2 && !rc; ++round) {
-        for (Cand& c : cands) {
-            plan->big_groups = c.groups;
-            plan->big_delay = c.delay;
-            for (int i = 0; i < 2 && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
-            if (rc) break;
-            if (hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
-            for (int i = 0; i < reps && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
+    // Back to back on one buffer, a transform of up to a few hundred MiB
+    // finds its data in the 256 MB last-level cache, and the schedule that
+    // wins there is not the one that wins on cold data (a caller's previous
+    // work has evicted it).  Each timed run therefore follows a write of a
+    // buffer of that size (when one can be allocated) and is timed alone.
+    const size_t flush_bytes = size_t(256) << 20;
+    void* flush = nullptr;
+    if (hipMalloc(&flush, flush_bytes) != hipSuccess) {
+        flush = nullptr;
+        (void)hipGetLastError();
+    }
+    // mean of five single cold executions after two warm-up ones
+    auto time_current = [&](float& ms) {
+        for (int i = 0; i < 2 && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
+        float total = 0.f;
+        for (int i = 0; i < 5 && !rc; ++i) {
+            if (flush && hipMemsetAsync(flush, i + 1, flush_bytes, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            if (!rc) rc = run_passes(plan, (uint32_t*)data, -1, st);
             if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
             if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
-            float ms = 0.f;
-            if (!rc && hipEventElapsedTime(&ms, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
-            if (rc) break;
-            c.ms = std::min(c.ms, ms / reps);
+            float t = 0.f;
+            if (!rc && hipEventElapsedTime(&t, a, b) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            total += t;
         }
+        ms = total / 5.f;
+    };
+    // Each candidate is timed once, then the four fastest again in three
+    // alternating rounds, each keeping its best: neighbouring schedules
+    // differ by a few percent, less than one sweep's noise.
+    struct Cand {
+        int a, b;  // (groups, delay) of the first pass, or (cols delay, 0)
+        float ms;
+    };
+    auto tune = [&](std::vector<Cand>& cands, const std::function<void(const Cand&)>& apply) {
+        for (Cand& c : cands) {
+            if (rc) return;
+            apply(c);
+            time_current(c.ms);
+        }
+        std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+        if (cands.size() > 4) cands.resize(4);
+        for (int round = 0; round < 3 && !rc; ++round)
+            for (Cand& c : cands) {
+                float ms = 0.f;
+                apply(c);
+                time_current(ms);
+                if (!rc) c.ms = std::min(c.ms, ms);
+            }
+        std::sort(cands.begin(), cands.end(), [](const Cand& x, const Cand& y) { return x.ms < y.ms; });
+    };
+    // start schedule of the first pass: no stagger, or 3, 5 or 7 groups
+    // (counts coprime with 2, see big_sched) with delays of 1..4 quanta
+    const int saved_groups = plan->big_groups, saved_delay = plan->big_delay;
+    std::vector<Cand> first{{3, 0, 0.f}};
+    for (int g : {3, 5, 7})
+        for (int d = 1; d <= 4; ++d) first.push_back({g, d, 0.f});
+    tune(first, [&](const Cand& c) {
+        plan->big_groups = c.a;
+        plan->big_delay = c.b;
+    });
+    float best_t = rc ? 0.f : first[0].ms;
+    if (rc) {
+        plan->big_groups = saved_groups;
+        plan->big_delay = saved_delay;
+    } else {
+        plan->big_groups = first[0].a;
+        plan->big_delay = first[0].b;
+        if (best_groups) *best_groups = first[0].a;
+        if (best_delay) *best_delay = first[0].b;
     }
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
-    plan->big_groups = saved_groups;
-    plan->big_delay = saved_delay;
-    if (rc) return rc;
-    const Cand* best = &cands[0];
-    for (const Cand& c : cands)
-        if (c.ms < best->ms) best = &c;
-    plan->big_groups = best->groups;
-    plan->big_delay = best->delay;
-    float best_t = best->ms;
-    if (best_groups) *best_groups = best->groups;
-    if (best_delay) *best_delay = best->delay;
     // then the start delay of a radix-256*W column pass, if the plan has one
     bool has_cols_big = false;
     for (const Pass& ps : plan->lay.passes) has_cols_big |= ps.kind == 3;
-    if (has_cols_big) {
+    if (has_cols_big && !rc) {
         const int saved = plan->cols_delay;
-        int best_cd = saved;
-        float best_cd_t = 1e30f;
-        hipEvent_t c0, c1;
-        if (hipEventCreate(&c0) != hipSuccess) return TCFFT_HIP_ERROR;
-        if (hipEventCreate(&c1) != hipSuccess) {
-            (void)hipEventDestroy(c0);
-            return TCFFT_HIP_ERROR;
-        }
-        for (int round = 0; round < 2 && !rc; ++round)
-            for (int cd = 0; cd <= 2 && !rc; ++cd) {
-                plan->cols_delay = cd;
-                for (int i = 0; i < 2 && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
-                if (!rc && hipEventRecord(c0, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
-                for (int i = 0; i < reps && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
-                if (!rc && hipEventRecord(c1, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
-                if (!rc && hipEventSynchronize(c1) != hipSuccess) rc = TCFFT_HIP_ERROR;
-                float ms = 0.f;
-                if (!rc && hipEventElapsedTime(&ms, c0, c1) != hipSuccess) rc = TCFFT_HIP_ERROR;
-                if (!rc && ms / reps < best_cd_t) {
-                    best_cd_t = ms / reps;
-                    best_cd = cd;
-                }
-            }
-        (void)hipEventDestroy(c0);
-        (void)hipEventDestroy(c1);
-        plan->cols_delay = rc ? saved : best_cd;
-        if (rc) return rc;
-        best_t = std::min(best_t, best_cd_t);
+        std::vector<Cand> cols;
+        for (int cd = 0; cd <= 4; ++cd) cols.push_back({cd, 0, 0.f});
+        tune(cols, [&](const Cand& c) { plan->cols_delay = c.a; });
+        plan->cols_delay = rc ? saved : cols[0].a;
+        if (!rc) best_t = std::min(best_t, cols[0].ms);
     }
+    if (flush) (void)hipFree(flush);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    if (rc) return rc;
     if (best_ms) *best_ms = best_t;
     return TCFFT_SUCCESS;
 }
@@ -917,6 +962,23 @@ int tcfft_exec_passes(tcfft_plan plan, void* data, int direction, int first, int
 
 int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction, void* stream) {
     if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    return tcfft::exec_natural_staged(plan, in, out, direction, stream, plan->workspace, plan->workspace_bytes);
+}
+
+}  // extern "C"
+
+namespace tcfft {
+
+size_t natural_stage_bytes(tcfft_plan plan, const void* in, const void* out) {
+    if (!valid(plan) || in != out || !natural_needs_stage(plan->lay)) return 0;
+    return plan->total * 4u;
+}
+
+int plan_device(tcfft_plan plan) { return valid(plan) ? plan->device : -1; }
+
+int exec_natural_staged(tcfft_plan plan, const void* in, void* out, int direction, void* stream, void* stage_buf,
+                        size_t stage_cap) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
     if (!in || !out || ((uintptr_t)in & 15u) || ((uintptr_t)out & 15u)) return TCFFT_INVALID_POINTER;
     if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
     DeviceGuard guard(plan->device);
@@ -924,18 +986,11 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
     const Layout& L0 = plan->lay;
     const Pass& first = L0.passes[0];
     const size_t bytes = plan->total * 4;
-    auto stage = [&](const uint32_t*& src) -> int {  // copy of the input in the plan's workspace
-        if (!plan->workspace) {
-            hipError_t e = hipMalloc(&plan->workspace, bytes);
-            if (e != hipSuccess) {
-                plan->workspace = nullptr;
-                return hip_fail(e);
-            }
-            plan->workspace_bytes = bytes;
-        }
-        hipError_t e = hipMemcpyAsync(plan->workspace, in, bytes, hipMemcpyDeviceToDevice, st);
+    auto stage = [&](const uint32_t*& src) -> int {  // copy of the input in the staging buffer
+        if (!stage_buf || stage_cap < bytes) return TCFFT_INVALID_PLAN;
+        hipError_t e = hipMemcpyAsync(stage_buf, in, bytes, hipMemcpyDeviceToDevice, st);
         if (e != hipSuccess) return hip_fail(e);
-        src = (const uint32_t*)plan->workspace;
+        src = (const uint32_t*)stage_buf;
         return TCFFT_SUCCESS;
     };
     // First pass over lines of up to 4096 points (k_rows) or of 8192..32768
@@ -984,6 +1039,10 @@ int tcfft_exec_natural(tcfft_plan plan, const void* in, void* out, int direction
     if (e != hipSuccess) return hip_fail(e);
     return run_passes(plan, (uint32_t*)out, direction, st);
 }
+
+}  // namespace tcfft
+
+extern "C" {
 
 int tcfft_permute_to_dropin(tcfft_plan plan, const void* in, void* out, int n_lines, void* stream) {
     if (!valid(plan)) return TCFFT_INVALID_PLAN;

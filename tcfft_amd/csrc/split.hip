@@ -81,7 +81,11 @@ __device__ __forceinline__ void dft16_split_b(const float* re, const float* im, 
 }
 
 // 2^(13 - e) for the wave-wide largest magnitude m = f * 2^e (f in [0.5, 1)),
-// and its inverse; 1 for m = 0.
+// and its inverse; 1 for m = 0.  s = 2^(sb - 127) is clamped to at most
+// 2^126 (sb <= 253) so that its inverse 2^(127 - sb), biased exponent
+// 254 - sb >= 1, is a normal float and s * inv == 1 exactly: a wave whose
+// largest component is below 2^-114 is scaled by 2^126 only (its scaled
+// values stay below 2^13 all the same).
 __device__ __forceinline__ void wave_scale(float m, float& s, float& inv) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) m = fmaxf(m, __shfl_xor(m, d));
@@ -90,9 +94,9 @@ __device__ __forceinline__ void wave_scale(float m, float& s, float& inv) {
         s = inv = 1.f;
         return;
     }
-    const int sb = min(max(266 - ex, 1), 254), ib = min(max(ex - 12, 1), 254);
+    const int sb = min(max(266 - ex, 1), 253);
     s = __uint_as_float((u32)sb << 23);
-    inv = __uint_as_float((u32)ib << 23);
+    inv = __uint_as_float((u32)(254 - sb) << 23);
 }
 
 // src and dst alias in pass 2 (in place on the intermediate), so neither is

@@ -32,22 +32,43 @@ spectrum in registers and in the drop-in's digit-reversed order.
 """
 from __future__ import annotations
 
+from collections import OrderedDict
+
 import torch
 
 from . import FORWARD, INVERSE, Plan1D, PlanR2C, gen_rev
 
-_PLANS: dict = {}
-_RPLANS: dict = {}
+# Plans are cached per (n, batch, device), least recently used first out:
+# a plan holds device tables (and, for some layouts, a workspace as large as
+# its data), so a caller cycling through batch sizes must not grow device
+# memory without bound.
+PLAN_CACHE_SIZE = 8
+_PLANS: OrderedDict = OrderedDict()
+_RPLANS: OrderedDict = OrderedDict()
+
+
+def _cached(cache: OrderedDict, key, make):
+    p = cache.get(key)
+    if p is not None:
+        cache.move_to_end(key)
+        return p
+    with torch.cuda.device(key[2]):
+        p = make()
+    cache[key] = p
+    while len(cache) > PLAN_CACHE_SIZE:
+        _, old = cache.popitem(last=False)
+        # work queued with the evicted plan may still be running on any stream
+        torch.cuda.synchronize(old.device_index)
+        old.destroy()
+    return p
 
 
 def _plan(n: int, batch: int, device: torch.device) -> Plan1D:
-    key = (n, batch, device.index)
-    p = _PLANS.get(key)
-    if p is None:
-        with torch.cuda.device(device):
-            p = Plan1D(n, batch)
-        _PLANS[key] = p
-    return p
+    def make():
+        p = Plan1D(n, batch)
+        p.device_index = device.index
+        return p
+    return _cached(_PLANS, (n, batch, device.index), make)
 
 
 def _check(x: torch.Tensor):
@@ -106,13 +127,11 @@ def ifft(x: torch.Tensor) -> torch.Tensor:
 
 
 def _rplan(n: int, batch: int, device: torch.device) -> PlanR2C:
-    key = (n, batch, device.index)
-    p = _RPLANS.get(key)
-    if p is None:
-        with torch.cuda.device(device):
-            p = PlanR2C(n, batch)
-        _RPLANS[key] = p
-    return p
+    def make():
+        p = PlanR2C(n, batch)
+        p.device_index = device.index
+        return p
+    return _cached(_RPLANS, (n, batch, device.index), make)
 
 
 def _r2c(x: torch.Tensor) -> torch.Tensor:
@@ -212,6 +231,11 @@ class _FusedConv(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, weight, want_filter_grad: bool):
         _check(x)
+        if weight.dim() != 3 or weight.shape[-1] != 2:
+            raise RuntimeError("expected a filter spectrum of shape [C, n, 2]")
+        if x.dim() < 3 or x.shape[-3] != weight.shape[0] or x.shape[-2] != weight.shape[1]:
+            raise RuntimeError(f"expected x of shape [..., C={weight.shape[0]}, n={weight.shape[1]}, 2], "
+                               f"got {tuple(x.shape)}")
         x = x.contiguous()
         n, C = x.shape[-2], weight.shape[0]
         signals = x.numel() // (2 * n)

@@ -199,3 +199,24 @@ def test_fftconv_full_shape_subset():
         # gx = F^H (conj(H) * F g) / n
         ref_gx = np.fft.ifft(np.fft.fft(gs) * np.conj(hc[c]))
         assert rel(to_c(x.grad[b, c]), ref_gx) <= TOL
+    # filter gradient at the full shape: gH[c] = (1/n) sum_b F(g_b,c) conj(F(x_b,c)),
+    # summed over all 64 batch entries, against FP64 for sampled channels
+    gw = layer.weight.grad.detach().cpu().double()
+    for c in (0, 130, 255):
+        X = np.fft.fft(to_c(x[:, c]), axis=-1)
+        G = np.fft.fft(to_c(g[:, c]), axis=-1)
+        ref_gh = (G * np.conj(X)).sum(axis=0) / n
+        got = gw[c, :, 0].numpy() + 1j * gw[c, :, 1].numpy()
+        err = rel(got, ref_gh)
+        note(f"  filter grad channel {c}: rel_l2 {err:.2e}")
+        assert err <= TOL
+
+
+def test_fftconv_rejects_mismatched_layout():
+    """The fused layer pairs signal [..., c, :] with filter c, so an input whose
+    channel axis does not match the filter is an error, not a silent pairing."""
+    layer = torch_fft.FFTConv1d(4, 4096, device="cuda")
+    for shape in ((2, 8, 4096), (8, 4096), (2, 4, 8192)):
+        x = torch.zeros(*shape, 2, dtype=torch.float16, device="cuda")
+        with pytest.raises(RuntimeError):
+            layer(x)

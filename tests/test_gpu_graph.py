@@ -1,5 +1,6 @@
 """hipGraph capture of the entry points that allocate nothing: the drop-in
-exec (1D and 2D), out-of-place natural order, the uneven batch, split
+exec (1D and 2D), natural order (out of place, and in place through the
+plan's workspace), the uneven batch, split
 precision, the real-input transforms and the fused convolution.  Each graph is replayed three times with
 eager work (input restore, an eager transform of another buffer) between the
 replays, and every replay must equal the eager result bit for bit."""
@@ -90,6 +91,27 @@ def test_graph_natural_out_of_place(n, batch):
 
     replay_check(lambda st: plan.exec_natural(src.data_ptr(), dst.data_ptr(), tcfft_amd.FORWARD, st), reset, dst,
                  want, lambda: plan.exec(torch.empty_like(x).copy_(x).data_ptr()))
+    plan.destroy()
+
+
+@pytest.mark.parametrize("shape", [(2, 512, 1024), (2, 1 << 20)])
+def test_graph_natural_in_place(shape):
+    """In-place natural order stages the input through a workspace that the
+    plan allocated at creation (2D, and 1D lines above 65536 points), so the
+    call allocates nothing and captures into a graph; every replay equals
+    the out-of-place result."""
+    note(f"[gpu] graph: natural order in place {shape}")
+    rng = np.random.default_rng(sum(shape))
+    plan = tcfft_amd.Plan2D(*shape[1:], shape[0]) if len(shape) == 3 else tcfft_amd.Plan1D(shape[1], shape[0])
+    x = fp16(rng, shape + (2,))
+    want = torch.empty_like(x)
+    plan.exec_natural(x.data_ptr(), want.data_ptr())
+    torch.cuda.synchronize()
+    b = x.clone()
+    scratch = torch.empty_like(x)
+    replay_check(lambda st: plan.exec_natural(b.data_ptr(), b.data_ptr(), tcfft_amd.FORWARD, st),
+                 lambda: b.copy_(x), b, want,
+                 lambda: plan.exec_natural(scratch.copy_(x).data_ptr(), scratch.data_ptr()))
     plan.destroy()
 
 

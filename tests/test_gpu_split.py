@@ -102,3 +102,19 @@ def test_split_start_delay_bit_identical():
         with pytest.raises(tcfft_amd.TcfftError):
             plan.set_option(tcfft_amd.OPT_SPLIT_DELAY, bad)
     plan.destroy()
+
+
+def test_split_tiny_inputs():
+    """Signals whose largest component is below 2^-114 take the clamped wave
+    scale (2^126); scale and inverse must still multiply to exactly 1, so
+    the result keeps the same per-signal bound as ordinary inputs."""
+    note("[gpu] split tiny inputs n=65536 batch=4")
+    rng = np.random.default_rng(43)
+    scale = np.array([1e-36, 3e-37, 5e-36, 1.0])[:, None]
+    x = ((rng.standard_normal((4, N)) + 1j * rng.standard_normal((4, N))) * scale).astype(np.complex64)
+    assert np.abs(x[:3]).max() < 2.0 ** -114
+    for direction, sign in ((tcfft_amd.FORWARD, -1), (tcfft_amd.INVERSE, +1)):
+        got = run_split(x, direction)
+        errs = per_signal_err(got, oracle.fft(x.astype(np.complex128), sign))
+        note(f"  dir={direction} per-signal rel_l2 {[f'{e:.2e}' for e in errs]}")
+        assert max(errs) <= TOL
