@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "libtcfft_oracle.so")
+# ORACLE_LIBRARY selects another build of the checker (the sanitizer build of
+# tests/run_sanitized.py); by default the gcc build of oracle/Makefile.
+_LIB_PATH = os.environ.get("ORACLE_LIBRARY") or os.path.join(_HERE, "_build", "libtcfft_oracle.so")
 _lib = None
 
 _i32p = ctypes.POINTER(ctypes.c_int)

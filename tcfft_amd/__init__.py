@@ -22,7 +22,9 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libtcfft.so")
+# TCFFT_LIBRARY selects another build of the same library (the host-sanitizer
+# build of tests/run_sanitized.py); by default the in-tree libtcfft.so.
+LIB_PATH = os.environ.get("TCFFT_LIBRARY") or os.path.join(_HERE, "libtcfft.so")
 
 FORWARD = -1
 INVERSE = +1
