@@ -67,10 +67,19 @@ def parse():
 # ----------------------------------------------------------------------------
 # timing helpers
 
+# GPU cycles of the spin queued before each timed graph replay (about
+# 0.1 ms): the replay is submitted while the GPU spins, so the timed region
+# (start event -> end event) holds the K steps back to back and not the
+# host-side launch latency of the graph, which at K = 20 adds ~1 us per step.
+SPIN_CYCLES = 250_000
+
+
 def graph_time(torch, launch, reps, replays=1):
     """ms per launch(i, stream): reps launches captured once into a hipGraph,
     replayed `replays` times between synchronizations (replay() runs on the
-    current stream, so the events are recorded there); median over replays."""
+    current stream, so the events are recorded there); median over replays.
+    Each replay is queued behind a short spin kernel (SPIN_CYCLES) that ends
+    before the start event."""
     g = torch.cuda.CUDAGraph()
     cs = torch.cuda.Stream()
     cs.wait_stream(torch.cuda.current_stream())
@@ -83,6 +92,7 @@ def graph_time(torch, launch, reps, replays=1):
     for _ in range(replays):
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
+        torch.cuda._sleep(SPIN_CYCLES)
         a.record(torch.cuda.current_stream())
         g.replay()
         b.record(torch.cuda.current_stream())
@@ -399,6 +409,10 @@ def bench_uneven(torch, tcfft_amd, reps):
             p.exec(base + 4 * off, tcfft_amd.FORWARD, st)
 
     ms_sizes = graph_time(torch, per_size, reps, 3)
+    # natural-order signals (tcfft_multi_exec_natural, out of place, the same schedule)
+    nat_out = torch.empty_like(bufs[0])
+    ms_nat = graph_time(torch, lambda i, st: mp.exec_natural(bufs[i % len(bufs)].data_ptr(), nat_out.data_ptr(),
+                                                               tcfft_amd.FORWARD, st), reps, 3)
     # accuracy on a sample of signals, digit-reversed per signal
     sample = [int(i) for i in rng.choice(n_sig, 16, replace=False)]
     check = torch.from_numpy(flat).cuda()
@@ -421,6 +435,7 @@ def bench_uneven(torch, tcfft_amd, reps):
     return {"ms": round(ms_multi, 5), "gflops": round(flops / (ms_multi * 1e-3) / 1e9, 1),
             "hbm_frac": round(8.0 * total / (ms_multi * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
             "ms_per_size_launches": round(ms_sizes, 5), "speedup_vs_per_size": round(ms_sizes / ms_multi, 3),
+            "ms_natural_order": round(ms_nat, 5),
             "signals": n_sig, "elements": total, "tasks": n_tasks, "autotuned_grid": g, "autotuned_order": o, "autotuned_start_delay": delay,
             "rel_l2_max_sample": max(errs)}
 

@@ -176,8 +176,9 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
                            void *stream);
 
 /* Plan options. TCFFT_OPT_SPLIT_CHUNK: signals per pass pair of
- * tcfft_exec_split (0 = whole batch; default 128), which bounds the
- * intermediate's footprint so the second pass reads it from cache.
+ * tcfft_exec_split (0 = whole batch, the default), which bounds the
+ * intermediate's footprint (smaller chunks keep it in the last-level cache
+ * but leave compute units idle; at 2^16 x 256 the whole batch is fastest).
  * TCFFT_OPT_BIG_DELAY (-1..64, default -1 = per-kernel default) and
  * TCFFT_OPT_BIG_GROUPS (1..8, default 3): start schedule of the first pass
  * (contiguous lines of up to 65536 points); the workgroups resident at
@@ -188,21 +189,27 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * for a strided pass of radix 512..2048.
  * TCFFT_OPT_SPLIT_DELAY (0..64, default 2): the same start delay (3 groups)
  * for the two passes of tcfft_exec_split.
+ * TCFFT_OPT_ROWS_WAVES (1, 2 or 4, default 4): waves per workgroup of a
+ * contiguous-row first pass (lines of 256..4096 points, one wave per 16
+ * tiles of 256 points), i.e. how the waves are grouped onto compute units.
  * The schedule never changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,
     TCFFT_OPT_BIG_DELAY = 2,
     TCFFT_OPT_BIG_GROUPS = 3,
     TCFFT_OPT_COLS_DELAY = 4,
-    TCFFT_OPT_SPLIT_DELAY = 5
+    TCFFT_OPT_SPLIT_DELAY = 5,
+    TCFFT_OPT_ROWS_WAVES = 6
 };
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
 /* Times the plan's forward transform under each start schedule candidate
  * of its first pass (no stagger, or 3, 5 or 7 groups with delays of 1..4
  * quanta) on `data` (which it overwrites with transforms of itself), keeps
- * the fastest in the plan and reports it; then the column pass delay (0..4)
- * when the plan has a strided pass of radix 512..2048.  Every timed run
+ * the fastest in the plan and reports it; then, with that schedule, the
+ * workgroup size of a contiguous-row first pass (TCFFT_OPT_ROWS_WAVES) and
+ * the column pass delay (0..4) when the plan has a strided pass of radix
+ * 512..2048.  Every timed run
  * follows a write of a 256 MiB buffer that the call allocates and frees
  * (skipped if the allocation fails), so that the data is timed cold, as a
  * caller's previous work leaves it; the four fastest candidates are timed
@@ -222,6 +229,15 @@ int tcfft_multi_create(tcfft_multi *plan, int n_signals, const int *sizes);
 int tcfft_multi_destroy(tcfft_multi plan);
 int tcfft_multi_info(tcfft_multi plan, int *n_tasks, long long *total_elements);
 int tcfft_multi_exec(tcfft_multi plan, void *data, int direction, void *stream);
+/* The same transforms on natural-order signals (no counterpart in the
+ * reference, whose callers permute on the host, tcfft_doit_half.cpp:40-46):
+ * signal i of `in` at its packed offset, in natural order, to the same place
+ * in `out`, natural order.  in == out (in place) is allowed; in and out must
+ * not otherwise overlap.  Bit-identical to tcfft_multi_exec on each signal
+ * permuted by its own plan's gen_rev.  One persistent launch (the plan's
+ * schedule); the digit reversal runs in the tasks' loads through LDS.
+ * Allocates nothing; graph-capturable. */
+int tcfft_multi_exec_natural(tcfft_multi plan, const void *in, void *out, int direction, void *stream);
 /* Schedule: grid = persistent workgroups (0 = one per task), order = task
  * order (0 largest sizes first, 1 interleaved, 2 smallest first). */
 int tcfft_multi_set_schedule(tcfft_multi plan, int grid, int order);

@@ -45,12 +45,14 @@ EXPORTS = (
     "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad", "tcfft_plan_autotune",
     "tcfft_plan_create_1d_split", "tcfft_plan_create_1d_tuned", "tcfft_query_1d_split",
     "tcfft_plan_create_r2c_1d", "tcfft_exec_r2c", "tcfft_exec_c2r", "tcfft_permute_to_dropin",
+    "tcfft_multi_exec_natural",
 )
 OPT_SPLIT_CHUNK = 1
 OPT_BIG_DELAY = 2
 OPT_BIG_GROUPS = 3
 OPT_COLS_DELAY = 4
 OPT_SPLIT_DELAY = 5
+OPT_ROWS_WAVES = 6
 
 
 class TcfftError(RuntimeError):
@@ -108,6 +110,7 @@ def lib():
         L.tcfft_multi_destroy.argtypes = [_vp]
         L.tcfft_multi_info.argtypes = [_vp, _i32p, ctypes.POINTER(ctypes.c_longlong)]
         L.tcfft_multi_exec.argtypes = [_vp, _vp, ctypes.c_int, _vp]
+        L.tcfft_multi_exec_natural.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_multi_set_schedule.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.tcfft_multi_autotune.argtypes = [_vp, _vp, _vp, _i32p, _i32p, ctypes.POINTER(ctypes.c_float)]
         L.tcfft_multi_set_start_delay.argtypes = [_vp, ctypes.c_int]
@@ -356,6 +359,12 @@ class MultiPlan:
 
     def exec(self, data_ptr: int, direction: int = FORWARD, stream: int = 0):
         _check(lib().tcfft_multi_exec(self._h, _vp(data_ptr), direction, _vp(stream)), "tcfft_multi_exec")
+
+    def exec_natural(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
+        """Natural-order signals in and out (in == out allowed); the same
+        bits as exec on each signal permuted by its own gen_rev."""
+        _check(lib().tcfft_multi_exec_natural(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)),
+               "tcfft_multi_exec_natural")
 
     def set_schedule(self, grid: int, order: int):
         _check(lib().tcfft_multi_set_schedule(self._h, grid, order), "tcfft_multi_set_schedule")

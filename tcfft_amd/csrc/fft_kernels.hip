@@ -60,10 +60,15 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
     __builtin_amdgcn_sched_barrier(0);
     u32 d[NT][4];
     if constexpr (std::is_same_v<Load, DefaultRowsLoad>) {
+        // Unconditional loads in tile order, so that they complete in that
+        // order and each tile's first stages wait for that tile only.  Absent
+        // tiles (t >= ntl, always whole lines: ntl is a multiple of T) load a
+        // copy of the last present tile; the lines they form are computed
+        // but never stored, and no line mixes present and absent tiles.
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
-            if ((u32)t < ntl) load4_nt(d[t], tile_ptr(t));
-            else d[t][0] = d[t][1] = d[t][2] = d[t][3] = 0u;
+            load4_nt(d[t], tile_ptr((u32)t < ntl ? t : (int)ntl - 1));
+            __builtin_amdgcn_sched_barrier(0);
         }
     } else {
         load(d);
@@ -653,19 +658,32 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_cols_q(u32* __restric
 //            W_{256W}^{k0*w}); transpose to tile = k0, (g,i) = j1;
 //   stage 1: DFT-16 over j1 whose operand rows are scaled by
 //            W_{256W}^{16*k1*w} -> tile k0, (g,i) = k1.
-// Exchange through LDS laid out [source wave][tile][lane][4 words] per round
-// (b128 both ways, conflict-free), each wave taking TPW tiles of every source
-// per round, then the radix-W stage over the sources (VALU) writes row
-// k0 + 16*k1 + 256*k2.  Workgroups are mapped so that those on one XCD
-// (block mod 8) take adjacent column groups, whose 64-byte row segments share
-// 128-byte lines in that XCD's L2.
+// Global accesses are 16-byte lanes: four lanes cover one row's 64-byte
+// segment, one instruction 16 rows.  Loads: wave w reads its 256 rows
+// (row-major, 16 words per row) and passes them through its own LDS slice
+// in two halves of 128 rows, reading back the B-form layout (lane (c16, g)
+// word i of tile j1 = row 16*j1 + 4g + i, column c16).  The exchange between
+// the radix-256 stages and the radix-W stage is laid out [source wave][tile]
+// [k1][16 columns] per round; each reader lane takes 4 adjacent columns of
+// one row k1 of a tile (a 16-byte read per source), computes the radix-W
+// stage over the sources for them and stores rows k0 + 16*k1 + 256*k2 as
+// 16-byte pieces.  Rows of 16 words are placed at slot r ^ ((r >> 2) & 3)
+// (within their group of 16), so the b32 accesses of the 4 row groups g of
+// a wave fall in distinct bank ranges and the 16-byte ones stay whole.
+// Workgroups are mapped so that those on one XCD (block mod 8) take
+// adjacent column groups, whose 64-byte row segments share 128-byte lines
+// in that XCD's L2.
 template <int W>
 struct ColsBigCfg {
     static constexpr int ROUNDS = W >= 4 ? 2 : 1;
     static constexpr int TPR = 16 / ROUNDS;  // tiles exchanged per round
     static constexpr int TPW = TPR / W;      // tiles per reader wave per round
-    static constexpr int LDS_WORDS = W * TPR * 256;
+    static constexpr int XCH_WORDS = W * TPR * 256;
+    static constexpr int STAGE_WORDS = W * 128 * 16;  // 128 rows of 16 words per wave
+    static constexpr int LDS_WORDS = XCH_WORDS > STAGE_WORDS ? XCH_WORDS : STAGE_WORDS;
 };
+
+__device__ __forceinline__ u32 cols_slot(u32 r) { return r ^ ((r >> 2) & 3u); }
 
 template <int W, int SIGN>
 __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __restrict__ buf, u32 s_log2,
@@ -685,7 +703,9 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
     const u32 cg_log2 = s_log2 - 4u;  // column groups of 16 per block: s / 16
     const u32 blk = grp >> cg_log2;
     const u32 col0 = (grp & ((1u << cg_log2) - 1u)) * 16u;
-    const size_t blk_base = (((size_t)blk * (256u * W)) << s_log2) + col0 + (u32)c16;
+    const size_t blk_base = (((size_t)blk * (256u * W)) << s_log2) + col0;
+    // 16-byte lane: row a of each group of 16 rows, columns 4q .. 4q + 3
+    const u32 a = (u32)lane >> 2, q = (u32)lane & 3u;
 
     // table values before the data (global loads complete in issue order)
     float2 t_step[4], t_rec0[4];
@@ -697,12 +717,26 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
     }
     const float2 t_s1 = w4096[((u32)c16 * w * (256u / W)) & 4095u];  // W_{256W}^{16*c16*w}
     __builtin_amdgcn_sched_barrier(0);
+    u32 v[16][4];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        load4_nt(v[k], buf + blk_base + ((size_t)(256u * w + 16u * k + a) << s_log2) + 4u * q);
+        __builtin_amdgcn_sched_barrier(0);  // keep the row order of the loads
+    }
+    // through this wave's LDS slice, 128 rows at a time
     u32 d[16][4];
+    u32* const slice = lds + w * (128u * 16u);
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
+    for (int h = 0; h < 2; ++h) {
 #pragma unroll
-        for (int i = 0; i < 4; ++i) d[t][i] = buf[blk_base + ((size_t)(16u * t + 4u * g + i + 256u * w) << s_log2)];
-    __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < 8; ++k) store4(slice + cols_slot(16u * k + a) * 16u + 4u * q, v[8 * h + k]);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) d[8 * h + t][i] = slice[cols_slot(16u * t + 4u * g + i) * 16u + c16];
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    }
 
     const Dft16Op op = make_dft16(lane, SIGN);
     {
@@ -728,34 +762,40 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
     const Dft16Op op1 = make_dft16_scaled(lane_id(), SIGN, cf{t_s1.x, sg * t_s1.y});
 #pragma unroll
     for (int r = 0; r < C::ROUNDS; ++r) {
-        if (r) __syncthreads();  // the previous round's reads are done
+        // every wave has read its staged rows (r = 0) or the previous round
+        // (r > 0) before the exchange writes over them
+        __syncthreads();
 #pragma unroll
         for (int tt = 0; tt < C::TPR; ++tt) {
             f32x4 re, im;
             dft16_b(e[r * C::TPR + tt], op1, re, im);
-            u32 q[4];
+            u32* const row0 = lds + (w * C::TPR + tt) * 256u + (u32)c16;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) q[i] = pack(cf{re[i], im[i]});
-            store4(lds + ((w * C::TPR + tt) * 64u + (u32)lane) * 4u, q);
+            for (int i = 0; i < 4; ++i) row0[cols_slot(4u * g + i) * 16u] = pack(cf{re[i], im[i]});
         }
         __syncthreads();
 #pragma unroll
         for (int ti = 0; ti < C::TPW; ++ti) {
             const u32 tt = w * C::TPW + ti;
-            u32 v[W][4];
+            u32 x4[W][4];
 #pragma unroll
-            for (int src = 0; src < W; ++src) load4(v[src], lds + (((u32)src * C::TPR + tt) * 64u + (u32)lane) * 4u);
+            for (int src = 0; src < W; ++src)
+                load4(x4[src], lds + ((u32)src * C::TPR + tt) * 256u + cols_slot(a) * 16u + 4u * q);
             const u32 k0 = (u32)r * C::TPR + tt;
+            u32 o[W][4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
+            for (int j = 0; j < 4; ++j) {
                 cf x[W];
 #pragma unroll
-                for (int src = 0; src < W; ++src) x[src] = unpack(v[src][i]);
+                for (int src = 0; src < W; ++src) x[src] = unpack(x4[src][j]);
                 small_dft<W>(x, SIGN);
-                const u32 row = k0 + 16u * (4u * g + i);
 #pragma unroll
-                for (int k2 = 0; k2 < W; ++k2) buf[blk_base + ((size_t)(row + 256u * k2) << s_log2)] = pack(x[k2]);
+                for (int k2 = 0; k2 < W; ++k2) o[k2][j] = pack(x[k2]);
             }
+            const u32 row = k0 + 16u * a;  // k1 = a
+#pragma unroll
+            for (int k2 = 0; k2 < W; ++k2)
+                store4_nt(buf + blk_base + ((size_t)(row + 256u * k2) << s_log2) + 4u * q, o[k2]);
         }
     }
 }
@@ -854,6 +894,15 @@ __device__ __forceinline__ u32 nat_big_word(u32 m) {
 // 4-5 (writes conflict-free, reads at most 2-way among the 32 active lanes).
 __device__ __forceinline__ u32 nat16_word(u32 n) {
     const u32 m = (n & 2047u) | ((n >> 12) << 11);
+    return m ^ (((m >> 8) & 3u) << 2) ^ (((m >> 13) & 3u) << 4);
+}
+
+// Word of natural index n in a staging round that holds the indices with
+// bit B of n equal to the round number: bit B removed, then bits 8-9 into
+// 2-3 and 13-14 into 4-5 (a bijection that keeps 16-byte chunks whole).
+template <int B>
+__device__ __forceinline__ u32 nat_split_word(u32 n) {
+    const u32 m = (n & ((1u << B) - 1u)) | ((n >> (B + 1)) << B);
     return m ^ (((m >> 8) & 3u) << 2) ^ (((m >> 13) & 3u) << 4);
 }
 
@@ -1147,37 +1196,38 @@ static KernelInfo kernel_info(K kernel, int threads, size_t lds) {
 
 template <int T, int SIGN>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, const float2* w4096, const EpiParams* epi, StartSched sched,
-                                hipStream_t st) {
+                                u32 wg, hipStream_t st) {
     const u32 waves = (n_tiles + 15) / 16;
-    const u32 blocks = (waves + 3) / 4;
+    const u32 blocks = (waves + wg - 1) / wg;
     if (sched.groups == 0) sched.delay = 0;
     if (epi) {
         auto k = k_rows<T, true, SIGN>;
-        if (sched.delay) sched.resident = kernel_info(k, 256, 0).resident;
-        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, st, buf, n_tiles, w4096, *epi, sched);
+        if (sched.delay) sched.resident = kernel_info(k, (int)(64 * wg), 0).resident;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * wg), 0, st, buf, n_tiles, w4096, *epi, sched);
     } else {
         auto k = k_rows<T, false, SIGN>;
-        if (sched.delay) sched.resident = kernel_info(k, 256, 0).resident;
-        hipLaunchKernelGGL(k, dim3(blocks), dim3(256), 0, st, buf, n_tiles, w4096, EpiParams{}, sched);
+        if (sched.delay) sched.resident = kernel_info(k, (int)(64 * wg), 0).resident;
+        hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * wg), 0, st, buf, n_tiles, w4096, EpiParams{}, sched);
     }
     return hipGetLastError();
 }
 
 template <int T>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
-                                StartSched sched, hipStream_t st) {
-    return sign < 0 ? launch_rows_t<T, -1>(buf, n_tiles, w4096, epi, sched, st)
-                    : launch_rows_t<T, +1>(buf, n_tiles, w4096, epi, sched, st);
+                                StartSched sched, u32 wg, hipStream_t st) {
+    return sign < 0 ? launch_rows_t<T, -1>(buf, n_tiles, w4096, epi, sched, wg, st)
+                    : launch_rows_t<T, +1>(buf, n_tiles, w4096, epi, sched, wg, st);
 }
 
 hipError_t launch_rows(int T, u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
-                       StartSched sched, hipStream_t st) {
+                       StartSched sched, u32 wg_waves, hipStream_t st) {
+    if (wg_waves != 1 && wg_waves != 2 && wg_waves != 4) return hipErrorInvalidValue;
     switch (T) {
-    case 1: return launch_rows_t<1>(buf, n_tiles, sign, w4096, epi, sched, st);
-    case 2: return launch_rows_t<2>(buf, n_tiles, sign, w4096, epi, sched, st);
-    case 4: return launch_rows_t<4>(buf, n_tiles, sign, w4096, epi, sched, st);
-    case 8: return launch_rows_t<8>(buf, n_tiles, sign, w4096, epi, sched, st);
-    case 16: return launch_rows_t<16>(buf, n_tiles, sign, w4096, epi, sched, st);
+    case 1: return launch_rows_t<1>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
+    case 2: return launch_rows_t<2>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
+    case 4: return launch_rows_t<4>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
+    case 8: return launch_rows_t<8>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
+    case 16: return launch_rows_t<16>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
     default: return hipErrorInvalidValue;
     }
 }
@@ -1502,6 +1552,206 @@ __global__ __launch_bounds__(1024) void k_multi(u32* __restrict__ buf, const Mul
         }
         __syncthreads();  // the next task reuses the LDS
     }
+}
+
+// ----------------------------------------------------------------------------
+// Natural-order uneven batch (k_multi_nat): the tasks of k_multi on
+// natural-order input, in -> out (in place allowed: each signal belongs to
+// one wave or group, which reads it completely before its first store).
+// Row classes of 256..2048 points pass each wave's lines through the wave's
+// own 8 KiB of LDS (words k*L + natural index for the k-th line of a call)
+// and read them back in the digit-reversed register layout (nat_rows_read's
+// formula); a 4096-point line passes in two rounds split by index bit 8.
+// Big classes stage every group's signal in two rounds split by index bit
+// 9 or 11 (W * 8 KiB per group per round, 128 KiB in all).
+
+template <int T, int NT>
+__device__ __forceinline__ void multi_nat_read(u32 (&d)[NT][4], const u32* wl, u32 ntl, int lane) {
+    constexpr u32 L = 256u * T;
+    const int c16 = lane & 15, g = lane >> 4;
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const u32 m = (u32)(t / T) * L + (u32)(t % T) + T * (u32)c16 + 16u * T * (4u * g + j);
+            d[t][j] = (u32)t < ntl ? wl[nat_lds_word(m)] : 0u;
+        }
+}
+
+template <int T, int SIGN>
+__device__ __forceinline__ void multi_rows_nat(const u32* in, u32* out, const MultiTask& tk,
+                                               const u32* __restrict__ offsets, const float2* w4096, u32* lds) {
+    constexpr int LPW = 16 / T;
+    constexpr u32 L = 256u * T;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = lane_id();
+    const int first_line = wave * LPW;
+    const int lines = min((int)tk.count - first_line, LPW);
+    const int c16 = lane & 15, g = lane >> 4;
+    const u32 lane_off = 16u * c16 + 4u * g;
+    const u32* offs = offsets + tk.first + first_line;
+    if constexpr (T == 16) {
+        if (lines <= 0) return;
+        // one 4096-point line per wave through the wave's 2048-word slice in
+        // two rounds: round r holds the natural indices with bit 8 = r, i.e.
+        // the loaded tiles with t & 1 = r and the read words with j & 1 = r
+        // (reader index t + 16*c16 + 256*(4g + j)), so that which registers
+        // each round writes and reads is fixed at compile time
+        u32* const wl = lds + (u32)wave * 2048u;
+        auto load = [&](u32 (&d)[16][4]) {
+            u32 v[16][4];
+            const u32* src = in + offs[0] + 4u * lane;
+#pragma unroll
+            for (int t = 0; t < 16; ++t) load4_nt(v[t], src + 256u * t);
+#pragma unroll
+            for (int r = 0; r < 2; ++r) {
+#pragma unroll
+                for (int tt = 0; tt < 8; ++tt)
+                    store4(wl + nat_split_word<8>(256u * (2u * tt + r) + 4u * lane), v[2 * tt + r]);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+#pragma unroll
+                for (int t = 0; t < 16; ++t)
+#pragma unroll
+                    for (int j = r; j < 4; j += 2)
+                        d[t][j] = wl[nat_split_word<8>((u32)t + 16u * c16 + 256u * (4u * g + j))];
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            }
+        };
+        rows_body<16, false, SIGN>([&](int t) { return out + offs[0] + 256u * t + lane_off; }, 16u, 0u, w4096,
+                                   EpiParams{}, load);
+    } else {
+        if (lines <= 0) return;
+        u32* const wl = lds + (u32)wave * 2048u;  // 8 KiB per wave: NT * 256 words
+        constexpr int NT = (T == 1 || T == 8) ? 8 : 4;  // as multi_rows
+        for (int t0 = 0; t0 < lines * T; t0 += NT) {
+            const u32 ntl = (u32)min(lines * T - t0, NT);
+            auto load = [&](u32 (&d)[NT][4]) {
+#pragma unroll
+                for (int k = 0; k < NT / T; ++k)
+                    if ((u32)(k * T) < ntl) {
+                        const u32* src = in + offs[t0 / T + k] + 4u * lane;
+#pragma unroll
+                        for (int tt = 0; tt < T; ++tt) {
+                            u32 v[4];
+                            load4_nt(v, src + 256u * tt);
+                            store4(wl + nat_lds_word((u32)k * L + 256u * tt + 4u * lane), v);
+                        }
+                    }
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+                multi_nat_read<T, NT>(d, wl, ntl, lane);
+                __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+            };
+            rows_body<T, false, SIGN, NT>(
+                [&](int t) { return out + offs[(t0 + t) / T] + 256u * ((t0 + t) % T) + lane_off; }, ntl, 0u, w4096,
+                EpiParams{}, load);
+        }
+    }
+}
+
+template <int W, int SIGN>
+__device__ __forceinline__ void multi_big_nat(const u32* in, u32* out, const MultiTask& tk,
+                                              const u32* __restrict__ offsets, const float2* w4096,
+                                              const MultiTables& tb, u32* lds) {
+    constexpr int M = W == 2 ? 13 : W == 4 ? 14 : W == 8 ? 15 : 16;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int grp = wave / W, w = wave % W;
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+    // a group without a signal (last task of a class) reads the task's first
+    // signal, so that every load stays in bounds, and stores nothing
+    const bool active = grp < (int)tk.count;
+    const u32 off = offsets[tk.first + (active ? grp : 0)];
+    u32 v[16][4];
+    {
+        const u32* src = in + off + 4096u * (u32)w + 4u * lane;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) load4_nt(v[t], src + 256u * t);
+    }
+    // Two rounds over every group at once: round r stages the natural
+    // indices n with bit B of n = r at word nat_split_word<B>(n) of the
+    // group's W * 2048 words.  B is chosen so that the writer's half is a
+    // set of its loaded tiles (n = 4096w + 256t + ...: tile bit B - 8) and,
+    // for W = 2, 4, 8, the reader's half a set of its registers (reader
+    // n = w + W*(t + 16*c16 + 256*(4g + j)): bit 9 = j & 1 for W = 2, bit
+    // 11 = j >> 1 for W = 4 and j & 1 for W = 8); for W = 16 bit 11 is
+    // c16 >> 3, a lane predicate.
+    constexpr int B = W == 2 ? 9 : 11;
+    u32 d[16][4];
+    u32* const region = lds + (u32)grp * (2048u * W);
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        if (r) __syncthreads();  // round 0's reads are done
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+            if (((t >> (B - 8)) & 1) == r)
+                store4(region + nat_split_word<B>(4096u * (u32)w + 256u * t + 4u * lane), v[t]);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const u32 n = (u32)w + W * ((u32)t + 16u * c16 + 256u * (4u * g + j));
+                if constexpr (W == 16) {
+                    if ((c16 >> 3) == r) d[t][j] = region[nat_split_word<B>(n)];
+                } else {
+                    if ((W == 4 ? (j >> 1) : (j & 1)) == r) d[t][j] = region[nat_split_word<B>(n)];
+                }
+            }
+    }
+    __syncthreads();  // every wave has read its data before the exchange reuses the LDS
+    BigTw bt;
+    big_tw_load<W, false>(bt, lane_id(), w, 0u, w4096, tb.hi[M - 13], tb.lo[M - 13], (u32)(M / 2), EpiParams{});
+    const Dft16Op op = make_dft16(lane_id(), SIGN);
+    u32 e[16][4];
+    big_phase1<SIGN>(d, e, bt, op);
+    big_phase2<W, 2, false, SIGN>(e, w, lds + grp * (W * 2048), out + off, bt, op, active);
+}
+
+template <int SIGN>
+__global__ __launch_bounds__(1024) void k_multi_nat(const u32* in, u32* out, const MultiTask* __restrict__ tasks,
+                                                    u32 n_tasks, const u32* __restrict__ offsets,
+                                                    const float2* __restrict__ w4096, MultiTables tb,
+                                                    StartSched sched) {
+    extern __shared__ u32 lds[];
+    start_delay(sched);
+    for (u32 ti = blockIdx.x; ti < n_tasks; ti += gridDim.x) {
+        const MultiTask tk = tasks[ti];
+        switch (tk.size_log2) {
+        case 8: multi_rows_nat<1, SIGN>(in, out, tk, offsets, w4096, lds); break;
+        case 9: multi_rows_nat<2, SIGN>(in, out, tk, offsets, w4096, lds); break;
+        case 10: multi_rows_nat<4, SIGN>(in, out, tk, offsets, w4096, lds); break;
+        case 11: multi_rows_nat<8, SIGN>(in, out, tk, offsets, w4096, lds); break;
+        case 12: multi_rows_nat<16, SIGN>(in, out, tk, offsets, w4096, lds); break;
+        case 13: multi_big_nat<2, SIGN>(in, out, tk, offsets, w4096, tb, lds); break;
+        case 14: multi_big_nat<4, SIGN>(in, out, tk, offsets, w4096, tb, lds); break;
+        case 15: multi_big_nat<8, SIGN>(in, out, tk, offsets, w4096, tb, lds); break;
+        case 16: multi_big_nat<16, SIGN>(in, out, tk, offsets, w4096, tb, lds); break;
+        default: break;
+        }
+        __syncthreads();  // the next task reuses the LDS
+    }
+}
+
+template <int SIGN>
+static hipError_t launch_multi_nat_t(const u32* in, u32* out, const MultiTask* tasks, u32 n_tasks,
+                                     const u32* offsets, u32 grid, const float2* w4096, const MultiTables& tb,
+                                     StartSched sched, hipStream_t st) {
+    constexpr size_t lds = 16 * 2048 * sizeof(u32);  // 128 KiB
+    auto k = k_multi_nat<SIGN>;
+    const KernelInfo ki = kernel_info(k, 1024, lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    if (sched.groups == 0) sched.delay = 0;
+    if (sched.delay) sched.resident = ki.resident;
+    hipLaunchKernelGGL(k, dim3(grid), dim3(1024), lds, st, in, out, tasks, n_tasks, offsets, w4096, tb, sched);
+    return hipGetLastError();
+}
+
+hipError_t launch_multi_nat(const u32* in, u32* out, const MultiTask* tasks, u32 n_tasks, const u32* offsets,
+                            u32 grid, int sign, const float2* w4096, const MultiTables& tb, StartSched sched,
+                            hipStream_t st) {
+    if (n_tasks == 0) return hipSuccess;
+    if (grid == 0 || grid > n_tasks) grid = n_tasks;
+    return sign < 0 ? launch_multi_nat_t<-1>(in, out, tasks, n_tasks, offsets, grid, w4096, tb, sched, st)
+                    : launch_multi_nat_t<+1>(in, out, tasks, n_tasks, offsets, grid, w4096, tb, sched, st);
 }
 
 template <int SIGN>

@@ -32,9 +32,10 @@ struct StartSched {
     uint32_t resident;
 };
 
-// First pass: contiguous lines of 256*T elements (T = 1, 2, 4, 8, 16).
+// First pass: contiguous lines of 256*T elements (T = 1, 2, 4, 8, 16), one
+// wave per 16 tiles, wg_waves (1, 2 or 4) waves per workgroup.
 hipError_t launch_rows(int T, uint32_t* buf, uint32_t n_tiles, int sign, const float2* w4096,
-                       const EpiParams* epi, StartSched sched, hipStream_t st);
+                       const EpiParams* epi, StartSched sched, uint32_t wg_waves, hipStream_t st);
 
 // Single-pass transform of N = 4096*W (W = 2, 4, 8, 16) per workgroup of W
 // waves.  wn_hi/wn_lo: two-level table of W_N (lo_bits split).
@@ -67,6 +68,10 @@ struct MultiTables {  // two-level W_N tables for N = 2^13 .. 2^16
 hipError_t launch_multi(uint32_t* buf, const MultiTask* tasks, uint32_t n_tasks, const uint32_t* offsets,
                         uint32_t grid, int sign, const float2* w4096, const MultiTables& tb, StartSched sched,
                         hipStream_t st);
+// The same tasks on natural-order input: in -> out (in == out allowed).
+hipError_t launch_multi_nat(const uint32_t* in, uint32_t* out, const MultiTask* tasks, uint32_t n_tasks,
+                            const uint32_t* offsets, uint32_t grid, int sign, const float2* w4096,
+                            const MultiTables& tb, StartSched sched, hipStream_t st);
 
 // Natural-order transform of N = 4096*W (W = 1, 2, 4) per signal whose output
 // is the spectrum in the digit-reversed order the DIT kernels consume.

@@ -207,10 +207,13 @@ struct tcfft_plan_s {
     void* workspace = nullptr;
     size_t workspace_bytes = 0;
     size_t total = 0;  // elements in the buffer
-    int split_chunk = 128;  // signals per pass pair in tcfft_exec_split
+    // signals per pass pair in tcfft_exec_split, 0 = the whole batch (2^16 x
+    // 256: 99.6 us whole against 112.9 us in chunks of 128 on one box)
+    int split_chunk = 0;
     int cus = 0;          // compute units of the plan's device
     int big_delay = -1;   // start schedule of the first-pass kernels: -1 = default
     int big_groups = 3;
+    int rows_waves = 4;   // waves per workgroup of the contiguous-row passes (1, 2, 4)
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
     int split_delay = 2;  // start delay of the split-precision passes (3 groups; 2^16 x 256: 110.6 -> 106.6 us)
     int real_n = 0;       // real-input plan: the real length (its complex layout has n = real_n / 2)
@@ -350,8 +353,8 @@ tcfft::StartSched big_sched(const tcfft_plan_s* p, int W) {
 // 10.0 -> 9.2 us); with more workgroups per CU they desynchronise by
 // themselves and a delay only costs time (1024 x 8192: 15.9 -> 16.2 us).
 tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps) {
-    const uint32_t blocks = ((ps.n_tiles + 15u) / 16u + 3u) / 4u;
-    const int delay = p->big_delay >= 0 ? p->big_delay : (blocks <= (uint32_t)p->cus ? 1 : 0);
+    const uint32_t waves = (ps.n_tiles + 15u) / 16u;  // at most one wave per SIMD: a delay of 1
+    const int delay = p->big_delay >= 0 ? p->big_delay : (waves <= 4u * (uint32_t)p->cus ? 1 : 0);
     return tcfft::StartSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
 }
 
@@ -361,7 +364,9 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
         const Pass& ps = p->lay.passes[i];
         const EpiParams* epi = ps.has_epi ? &ps.epi : nullptr;
         hipError_t e;
-        if (ps.kind == 0) e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, rows_sched(p, ps), st);
+        if (ps.kind == 0)
+            e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, rows_sched(p, ps),
+                                   (uint32_t)p->rows_waves, st);
         else if (ps.kind == 2)
             e = tcfft::launch_big(ps.radix, buf, ps.n_tiles, sign, p->w4096, ps.big_hi, ps.big_lo, ps.big_M_log2 / 2,
                                   epi, big_sched(p, ps.radix), st);
@@ -724,12 +729,13 @@ int tcfft_exec_split(tcfft_plan plan, const void* in, void* out, int direction, 
     std::pair<float2*, float2*> t;
     int rc = table_pair(plan, 16, t);
     if (rc) return rc;
-    hipError_t e = tcfft::launch_split_65536((const float2*)in, (float2*)out, (uint32_t)L.batch,
-                                             (uint32_t)plan->split_chunk, direction, plan->w4096, t.first, t.second,
-                                             tcfft::StartSched{(uint32_t)plan->split_delay, 3u, 0u},
-                                             (hipStream_t)stream);
-    if (e != hipSuccess) return hip_fail(e);
-    return TCFFT_SUCCESS;
+    const tcfft::StartSched sched{(uint32_t)plan->split_delay, 3u, 0u};
+    hipStream_t st = (hipStream_t)stream;
+    const uint32_t batch = (uint32_t)L.batch;
+    const uint32_t chunk = plan->split_chunk <= 0 || plan->split_chunk > (int)batch ? batch : (uint32_t)plan->split_chunk;
+    const hipError_t e = tcfft::launch_split_65536((const float2*)in, (float2*)out, batch, chunk, direction,
+                                                   plan->w4096, t.first, t.second, sched, st);
+    return e == hipSuccess ? TCFFT_SUCCESS : hip_fail(e);
 }
 
 }  // extern "C"
@@ -836,6 +842,10 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
         if (value < 0 || value > 64) return TCFFT_INVALID_VALUE;
         plan->split_delay = value;
         return TCFFT_SUCCESS;
+    case TCFFT_OPT_ROWS_WAVES:
+        if (value != 1 && value != 2 && value != 4) return TCFFT_INVALID_VALUE;
+        plan->rows_waves = value;
+        return TCFFT_SUCCESS;
     default: return TCFFT_INVALID_VALUE;
     }
 }
@@ -923,6 +933,15 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
         plan->big_delay = first[0].b;
         if (best_groups) *best_groups = first[0].a;
         if (best_delay) *best_delay = first[0].b;
+    }
+    // then the workgroup size of a contiguous-row first pass (1, 2 or 4
+    // waves), with the chosen start schedule
+    if (!rc && plan->lay.passes[0].kind == 0) {
+        const int saved = plan->rows_waves;
+        std::vector<Cand> wg{{4, 0, 0.f}, {2, 0, 0.f}, {1, 0, 0.f}};
+        tune(wg, [&](const Cand& c) { plan->rows_waves = c.a; });
+        plan->rows_waves = rc ? saved : wg[0].a;
+        if (!rc) best_t = std::min(best_t, wg[0].ms);
     }
     // then the start delay of a radix-256*W column pass, if the plan has one
     bool has_cols_big = false;
@@ -1292,6 +1311,22 @@ int tcfft_multi_exec(tcfft_multi plan, void* data, int direction, void* stream) 
     hipError_t e = tcfft::launch_multi((uint32_t*)data, plan->d_tasks, (uint32_t)plan->tasks.size(), plan->d_offsets,
                                        plan->grid, direction, plan->w4096, plan->tb,
                                        tcfft::StartSched{(uint32_t)plan->delay, 3u, 0u}, (hipStream_t)stream);
+    if (e != hipSuccess) return hip_fail(e);
+    return TCFFT_SUCCESS;
+}
+
+int tcfft_multi_exec_natural(tcfft_multi plan, const void* in, void* out, int direction, void* stream) {
+    if (!valid_multi(plan)) return TCFFT_INVALID_PLAN;
+    if (!in || !out || ((uintptr_t)in & 15u) || ((uintptr_t)out & 15u)) return TCFFT_INVALID_POINTER;
+    const uintptr_t a = (uintptr_t)in, b = (uintptr_t)out;
+    const size_t bytes = plan->total * 4u;
+    if (in != out && a < b + bytes && b < a + bytes) return TCFFT_INVALID_POINTER;  // partial overlap
+    if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
+    DeviceGuard guard(plan->device);
+    hipError_t e = tcfft::launch_multi_nat((const uint32_t*)in, (uint32_t*)out, plan->d_tasks,
+                                           (uint32_t)plan->tasks.size(), plan->d_offsets, plan->grid, direction,
+                                           plan->w4096, plan->tb, tcfft::StartSched{(uint32_t)plan->delay, 3u, 0u},
+                                           (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e);
     return TCFFT_SUCCESS;
 }

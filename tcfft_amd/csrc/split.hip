@@ -102,22 +102,27 @@ __device__ __forceinline__ void wave_scale(float m, float& s, float& inv) {
 // src and dst alias in pass 2 (in place on the intermediate), so neither is
 // __restrict__: a workgroup reads its 64 columns completely before its first
 // store, and workgroups own disjoint columns.
-template <int PASS, int SIGN>
-__global__ __launch_bounds__(256, 2) void k_split_col(const float2* src, float2* dst,
-                                                   const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
-                                                   const float2* __restrict__ wn_lo, StartSched sched) {
-    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
-    // start schedule (as the first-pass kernels): the resident workgroups in
-    // `groups` groups by their index within an XCD, group k sleeping k * delay
-    // quanta of 1024 cycles, so that loads, compute and stores overlap
+// Start schedule (as the first-pass kernels): the resident workgroups in
+// `groups` groups by their index within an XCD, group k sleeping k * delay
+// quanta of 1024 cycles, so that loads, compute and stores overlap.
+__device__ __forceinline__ void split_start_delay(const StartSched& sched) {
     if (sched.delay && blockIdx.x < sched.resident) {
         const u32 quanta = ((blockIdx.x >> 3) % sched.groups) * sched.delay;
         for (u32 i = 0; i < quanta; ++i) __builtin_amdgcn_s_sleep(16);
     }
+}
+
+// One workgroup's share of pass PASS: block blk = 4 * signal + column group
+// (64 columns of the signal's 256 x 256 matrix).
+template <int PASS, int SIGN>
+__device__ __forceinline__ void split_col_body(const float2* src, float2* dst, u32 blk,
+                                               const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
+                                               const float2* __restrict__ wn_lo) {
+    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
     const int lane = lane_id(), w = threadIdx.x >> 6;
     const int c16 = lane & 15, g = lane >> 4;
-    const u32 sig = blockIdx.x >> 2;
-    const u32 col = 64u * (blockIdx.x & 3u) + 16u * w + c16;
+    const u32 sig = blk >> 2;
+    const u32 col = 64u * (blk & 3u) + 16u * w + c16;
     const size_t sbase = (size_t)sig * 65536u;
 
     // the stage-1 twiddle step W_256^{4g+i} first (global loads complete in
@@ -265,6 +270,14 @@ __global__ __launch_bounds__(256, 2) void k_split_col(const float2* src, float2*
     }
 }
 
+template <int PASS, int SIGN>
+__global__ __launch_bounds__(256, 2) void k_split_col(const float2* src, float2* dst,
+                                                   const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
+                                                   const float2* __restrict__ wn_lo, StartSched sched) {
+    split_start_delay(sched);
+    split_col_body<PASS, SIGN>(src, dst, blockIdx.x, w4096, wn_hi, wn_lo);
+}
+
 // Workgroups of 256 threads of kernel k resident at once on the current device.
 template <class K>
 static u32 split_resident(K k) {
@@ -285,12 +298,12 @@ static hipError_t launch_split_t(const float2* in, float2* out, u32 n_signals, u
     static const u32 res2 = split_resident(k_split_col<2, SIGN>);
     if (sched.groups == 0) sched.delay = 0;
     if (chunk == 0 || chunk > n_signals) chunk = n_signals;
+    StartSched s1 = sched, s2 = sched;
+    s1.resident = res1;
+    s2.resident = res2;
     for (u32 s0 = 0; s0 < n_signals; s0 += chunk) {
         const u32 cnt = min(chunk, n_signals - s0);
         const size_t off = (size_t)s0 * 65536;
-        StartSched s1 = sched, s2 = sched;
-        s1.resident = res1;
-        s2.resident = res2;
         hipLaunchKernelGGL((k_split_col<1, SIGN>), dim3(cnt * 4), dim3(256), 0, st, in + off, out + off, w4096, wn_hi,
                            wn_lo, s1);
         hipLaunchKernelGGL((k_split_col<2, SIGN>), dim3(cnt * 4), dim3(256), 0, st, out + off, out + off, w4096, wn_hi,

@@ -132,10 +132,31 @@ def test_graph_uneven_batch():
     mp.destroy()
 
 
-def test_graph_split_precision():
-    note("[gpu] graph: split precision 2^16 x 8")
+def test_graph_uneven_batch_natural():
+    note("[gpu] graph: uneven batch, natural order")
+    rng = np.random.default_rng(6)
+    sizes = (2 ** rng.integers(8, 17, 90)).astype(np.int32)
+    mp = tcfft_amd.MultiPlan(sizes)
+    total = int(sizes.sum())
+    x = fp16(rng, (total, 2))
+    want = torch.empty_like(x)
+    mp.exec_natural(x.data_ptr(), want.data_ptr())
+    torch.cuda.synchronize()
+    out = torch.empty_like(x)
+    scratch = torch.empty_like(x)
+    replay_check(lambda st: mp.exec_natural(x.data_ptr(), out.data_ptr(), tcfft_amd.FORWARD, st),
+                 lambda: out.zero_(), out, want, lambda: mp.exec_natural(x.data_ptr(), scratch.data_ptr()))
+    mp.destroy()
+
+
+@pytest.mark.parametrize("chunk", [0, 3])
+def test_graph_split_precision(chunk):
+    """chunk 3 of 8 signals: three pass pairs (the last one ragged);
+    chunk 0: the whole batch in one pass pair."""
+    note(f"[gpu] graph: split precision 2^16 x 8, chunk {chunk}")
     rng = np.random.default_rng(7)
     plan = tcfft_amd.Plan1D(1 << 16, 8)
+    plan.set_option(tcfft_amd.OPT_SPLIT_CHUNK, chunk)
     x = torch.from_numpy(rng.standard_normal((8, 1 << 16, 2)).astype(np.float32)).cuda()
     want = torch.empty_like(x)
     plan.exec_split(x.data_ptr(), want.data_ptr())

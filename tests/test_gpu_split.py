@@ -118,3 +118,4 @@ def test_split_tiny_inputs():
         errs = per_signal_err(got, oracle.fft(x.astype(np.complex128), sign))
         note(f"  dir={direction} per-signal rel_l2 {[f'{e:.2e}' for e in errs]}")
         assert max(errs) <= TOL
+
