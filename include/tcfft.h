@@ -107,6 +107,20 @@ int tcfft_exec(tcfft_plan plan, void *data, int direction, void *stream);
  * case can be captured into a hipGraph. */
 int tcfft_exec_natural(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
 
+/* tcfft_exec_natural that allocates nothing and uses no plan workspace (no
+ * counterpart in the reference): for callers that cannot allocate device
+ * memory -- code under hipGraph capture, serving loops, harnesses that
+ * forbid hipMalloc.  Same arguments, kernels and results (bit for bit) as
+ * tcfft_exec_natural.  Its in-place cases (2D plans; 1D plans whose first
+ * pass does not read natural order, e.g. n > 65536) stage the input in a
+ * fixed device array of TCFFT_NOALLOC_SCRATCH_BYTES per GPU that the library
+ * holds until the process ends; a call that would stage more returns
+ * TCFFT_INVALID_SIZE.  All calls on one GPU share that array, so calls that
+ * overlap on one GPU (several streams or host threads) race on it and must
+ * be serialised by the caller.  No allocation, free or memset. */
+#define TCFFT_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int tcfft_exec_natural_noalloc(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
+
 /* The caller-side staging of tcfftExec on the device: for a 1D plan of n
  * points, out[b*n + p] = in[b*n + rev[p]] for n_lines lines (rev = the
  * plan's gen_rev order, tcfft_doit_half.cpp:6-29,40-46), as one LDS-tiled

@@ -45,8 +45,9 @@ EXPORTS = (
     "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad", "tcfft_plan_autotune",
     "tcfft_plan_create_1d_split", "tcfft_plan_create_1d_tuned", "tcfft_query_1d_split",
     "tcfft_plan_create_r2c_1d", "tcfft_exec_r2c", "tcfft_exec_c2r", "tcfft_permute_to_dropin",
-    "tcfft_multi_exec_natural",
+    "tcfft_exec_natural_noalloc", "tcfft_multi_exec_natural",
 )
+NOALLOC_SCRATCH_BYTES = 64 << 20  # TCFFT_NOALLOC_SCRATCH_BYTES
 OPT_SPLIT_CHUNK = 1
 OPT_BIG_DELAY = 2
 OPT_BIG_GROUPS = 3
@@ -89,6 +90,7 @@ def lib():
                                      _i32p, ctypes.c_int]
         L.tcfft_exec.argtypes = [_vp, _vp, ctypes.c_int, _vp]
         L.tcfft_exec_natural.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+        L.tcfft_exec_natural_noalloc.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
         L.tcfft_gen_rev.argtypes = [ctypes.c_int, _i32p, ctypes.c_int, _i32p]
         L.tcfft_plan_num_passes.argtypes = [_vp, _i32p]
         L.tcfft_exec_passes.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
@@ -200,6 +202,13 @@ class _Plan:
     def exec_natural(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
         _check(lib().tcfft_exec_natural(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)),
                "tcfft_exec_natural")
+
+    def exec_natural_noalloc(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
+        """exec_natural that allocates nothing: in-place staging goes to the
+        library's fixed per-GPU scratch (NOALLOC_SCRATCH_BYTES), shared by
+        all calls on that GPU."""
+        _check(lib().tcfft_exec_natural_noalloc(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)),
+               "tcfft_exec_natural_noalloc")
 
     def permute_to_dropin(self, in_ptr: int, out_ptr: int, n_lines: int, stream: int = 0):
         """out[b, p] = in[b, rev[p]] for n_lines lines of the plan's n (1D):

@@ -11,7 +11,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libtcfft.so")
-SOURCES = ["fft_kernels.hip", "conv.hip", "gather.hip", "split.hip", "plan.cpp"]
+SOURCES = ["fft_kernels.hip", "conv.hip", "gather.hip", "split.hip", "noalloc.hip", "plan.cpp"]
 
 # Packed FP32 (v_pk_*_f32) is disabled for all device code.
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
