@@ -658,32 +658,19 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_cols_q(u32* __restric
 //            W_{256W}^{k0*w}); transpose to tile = k0, (g,i) = j1;
 //   stage 1: DFT-16 over j1 whose operand rows are scaled by
 //            W_{256W}^{16*k1*w} -> tile k0, (g,i) = k1.
-// Global accesses are 16-byte lanes: four lanes cover one row's 64-byte
-// segment, one instruction 16 rows.  Loads: wave w reads its 256 rows
-// (row-major, 16 words per row) and passes them through its own LDS slice
-// in two halves of 128 rows, reading back the B-form layout (lane (c16, g)
-// word i of tile j1 = row 16*j1 + 4g + i, column c16).  The exchange between
-// the radix-256 stages and the radix-W stage is laid out [source wave][tile]
-// [k1][16 columns] per round; each reader lane takes 4 adjacent columns of
-// one row k1 of a tile (a 16-byte read per source), computes the radix-W
-// stage over the sources for them and stores rows k0 + 16*k1 + 256*k2 as
-// 16-byte pieces.  Rows of 16 words are placed at slot r ^ ((r >> 2) & 3)
-// (within their group of 16), so the b32 accesses of the 4 row groups g of
-// a wave fall in distinct bank ranges and the 16-byte ones stay whole.
-// Workgroups are mapped so that those on one XCD (block mod 8) take
-// adjacent column groups, whose 64-byte row segments share 128-byte lines
-// in that XCD's L2.
+// Exchange through LDS laid out [source wave][tile][lane][4 words] per round
+// (b128 both ways, conflict-free), each wave taking TPW tiles of every source
+// per round, then the radix-W stage over the sources (VALU) writes row
+// k0 + 16*k1 + 256*k2.  Workgroups are mapped so that those on one XCD
+// (block mod 8) take adjacent column groups, whose 64-byte row segments share
+// 128-byte lines in that XCD's L2.
 template <int W>
 struct ColsBigCfg {
     static constexpr int ROUNDS = W >= 4 ? 2 : 1;
     static constexpr int TPR = 16 / ROUNDS;  // tiles exchanged per round
     static constexpr int TPW = TPR / W;      // tiles per reader wave per round
-    static constexpr int XCH_WORDS = W * TPR * 256;
-    static constexpr int STAGE_WORDS = W * 128 * 16;  // 128 rows of 16 words per wave
-    static constexpr int LDS_WORDS = XCH_WORDS > STAGE_WORDS ? XCH_WORDS : STAGE_WORDS;
+    static constexpr int LDS_WORDS = W * TPR * 256;
 };
-
-__device__ __forceinline__ u32 cols_slot(u32 r) { return r ^ ((r >> 2) & 3u); }
 
 template <int W, int SIGN>
 __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __restrict__ buf, u32 s_log2,
@@ -703,9 +690,7 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
     const u32 cg_log2 = s_log2 - 4u;  // column groups of 16 per block: s / 16
     const u32 blk = grp >> cg_log2;
     const u32 col0 = (grp & ((1u << cg_log2) - 1u)) * 16u;
-    const size_t blk_base = (((size_t)blk * (256u * W)) << s_log2) + col0;
-    // 16-byte lane: row a of each group of 16 rows, columns 4q .. 4q + 3
-    const u32 a = (u32)lane >> 2, q = (u32)lane & 3u;
+    const size_t blk_base = (((size_t)blk * (256u * W)) << s_log2) + col0 + (u32)c16;
 
     // table values before the data (global loads complete in issue order)
     float2 t_step[4], t_rec0[4];
@@ -717,26 +702,12 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
     }
     const float2 t_s1 = w4096[((u32)c16 * w * (256u / W)) & 4095u];  // W_{256W}^{16*c16*w}
     __builtin_amdgcn_sched_barrier(0);
-    u32 v[16][4];
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        load4_nt(v[k], buf + blk_base + ((size_t)(256u * w + 16u * k + a) << s_log2) + 4u * q);
-        __builtin_amdgcn_sched_barrier(0);  // keep the row order of the loads
-    }
-    // through this wave's LDS slice, 128 rows at a time
     u32 d[16][4];
-    u32* const slice = lds + w * (128u * 16u);
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int t = 0; t < 16; ++t)
 #pragma unroll
-        for (int k = 0; k < 8; ++k) store4(slice + cols_slot(16u * k + a) * 16u + 4u * q, v[8 * h + k]);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-#pragma unroll
-        for (int t = 0; t < 8; ++t)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) d[8 * h + t][i] = slice[cols_slot(16u * t + 4u * g + i) * 16u + c16];
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    }
+        for (int i = 0; i < 4; ++i) d[t][i] = buf[blk_base + ((size_t)(16u * t + 4u * g + i + 256u * w) << s_log2)];
+    __builtin_amdgcn_sched_barrier(0);
 
     const Dft16Op op = make_dft16(lane, SIGN);
     {
@@ -762,40 +733,34 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
     const Dft16Op op1 = make_dft16_scaled(lane_id(), SIGN, cf{t_s1.x, sg * t_s1.y});
 #pragma unroll
     for (int r = 0; r < C::ROUNDS; ++r) {
-        // every wave has read its staged rows (r = 0) or the previous round
-        // (r > 0) before the exchange writes over them
-        __syncthreads();
+        if (r) __syncthreads();  // the previous round's reads are done
 #pragma unroll
         for (int tt = 0; tt < C::TPR; ++tt) {
             f32x4 re, im;
             dft16_b(e[r * C::TPR + tt], op1, re, im);
-            u32* const row0 = lds + (w * C::TPR + tt) * 256u + (u32)c16;
+            u32 q[4];
 #pragma unroll
-            for (int i = 0; i < 4; ++i) row0[cols_slot(4u * g + i) * 16u] = pack(cf{re[i], im[i]});
+            for (int i = 0; i < 4; ++i) q[i] = pack(cf{re[i], im[i]});
+            store4(lds + ((w * C::TPR + tt) * 64u + (u32)lane) * 4u, q);
         }
         __syncthreads();
 #pragma unroll
         for (int ti = 0; ti < C::TPW; ++ti) {
             const u32 tt = w * C::TPW + ti;
-            u32 x4[W][4];
+            u32 v[W][4];
 #pragma unroll
-            for (int src = 0; src < W; ++src)
-                load4(x4[src], lds + ((u32)src * C::TPR + tt) * 256u + cols_slot(a) * 16u + 4u * q);
+            for (int src = 0; src < W; ++src) load4(v[src], lds + (((u32)src * C::TPR + tt) * 64u + (u32)lane) * 4u);
             const u32 k0 = (u32)r * C::TPR + tt;
-            u32 o[W][4];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
+            for (int i = 0; i < 4; ++i) {
                 cf x[W];
 #pragma unroll
-                for (int src = 0; src < W; ++src) x[src] = unpack(x4[src][j]);
+                for (int src = 0; src < W; ++src) x[src] = unpack(v[src][i]);
                 small_dft<W>(x, SIGN);
+                const u32 row = k0 + 16u * (4u * g + i);
 #pragma unroll
-                for (int k2 = 0; k2 < W; ++k2) o[k2][j] = pack(x[k2]);
+                for (int k2 = 0; k2 < W; ++k2) buf[blk_base + ((size_t)(row + 256u * k2) << s_log2)] = pack(x[k2]);
             }
-            const u32 row = k0 + 16u * a;  // k1 = a
-#pragma unroll
-            for (int k2 = 0; k2 < W; ++k2)
-                store4_nt(buf + blk_base + ((size_t)(row + 256u * k2) << s_log2) + 4u * q, o[k2]);
         }
     }
 }

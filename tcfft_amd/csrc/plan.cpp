@@ -213,7 +213,10 @@ struct tcfft_plan_s {
     int cus = 0;          // compute units of the plan's device
     int big_delay = -1;   // start schedule of the first-pass kernels: -1 = default
     int big_groups = 3;
-    int rows_waves = 4;   // waves per workgroup of the contiguous-row passes (1, 2, 4)
+    // waves per workgroup of the contiguous-row passes (1, 2, 4); one-wave
+    // workgroups measured fastest (4096 x 1024: 7.2 us against 8.1 with 4,
+    // 1024-point rows of 2D 1024^2 x 8: 14.7 against 15.1)
+    int rows_waves = 1;
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
     int split_delay = 2;  // start delay of the split-precision passes (3 groups; 2^16 x 256: 110.6 -> 106.6 us)
     int real_n = 0;       // real-input plan: the real length (its complex layout has n = real_n / 2)
