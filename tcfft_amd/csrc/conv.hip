@@ -189,13 +189,19 @@ __device__ __forceinline__ void stepa_exchange(u32 (&v)[16][4], int w, int lane,
         const u32 m = (u32)(S * w + s);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            cf x[W];
-#pragma unroll
-            for (int n2 = 0; n2 < W; ++n2) x[n2] = unpack(v[s * W + n2][q]);
-            if constexpr (W > 1) small_dft<W>(x, SIGN);
             const u32 zp = dif_zpos(256u * m + 4u * lane + q);
+            if constexpr (W == 2) {
+                // radix 2 on the packed words (packed fp16 add and subtract)
+                lds[zp] = hadd2(v[s * W][q], v[s * W + 1][q]);
+                lds[4096u + zp] = hsub2(v[s * W][q], v[s * W + 1][q]);
+            } else {
+                cf x[W];
 #pragma unroll
-            for (int k = 0; k < W; ++k) lds[4096u * k + zp] = pack(x[k]);
+                for (int n2 = 0; n2 < W; ++n2) x[n2] = unpack(v[s * W + n2][q]);
+                if constexpr (W > 1) small_dft<W>(x, SIGN);
+#pragma unroll
+                for (int k = 0; k < W; ++k) lds[4096u * k + zp] = pack(x[k]);
+            }
         }
     }
 }
@@ -245,8 +251,8 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_dif(const u32* in,
 // fp16 complex in the digit-reversed spectrum order (conjugated with CONJ);
 // with SPEC the spectrum F x is also stored, digit-reversed, at spec.
 // The product is taken on the FP32 results of the last DIF stage; the
-// inverse is the DIT of the single-pass kernel (sign +1) and the scale is
-// applied to its FP32 results.
+// inverse is the DIT of the single-pass kernel (sign +1), with the scale
+// carried by its stage-1 twiddle recurrence (FP32, before rounding).
 // x and y may alias (in place), so neither is __restrict__: a workgroup reads
 // its signal completely before its first store of y.
 template <int W, bool CONJ, bool SPEC>
@@ -296,9 +302,9 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_conv(const u32* x,
     big_tw_load<W, false>(bt, lane, w, sig, w4096, wn_hi, wn_lo, wn_lo_bits, EpiParams{});
     const Dft16Op opi = make_dft16(lane, +1);
     u32 e[16][4];
-    big_phase1<+1>(d, e, bt, opi);
+    big_phase1<+1>(d, e, bt, opi, scale);  // the 1/n scale rides on the stage-1 recurrence
     __syncthreads();  // every wave has read its step-B line before the exchange reuses the LDS
-    big_phase2<W, 1, false, +1, true>(e, w, lds, y + (size_t)sig * N, bt, opi, true, scale);
+    big_phase2<W, 1, false, +1>(e, w, lds, y + (size_t)sig * N, bt, opi, true);
 }
 
 // ----------------------------------------------------------------------------

@@ -219,11 +219,18 @@ __device__ __forceinline__ cf raw_q(const RawQ& r) {
 // reversed chunk 4096*w .. 4096*w + 4095 of the signal, loaded as tile t =
 // d2, lane (c16 = d1, g), word j = d0), times W_N^{w*c}; the result is left
 // in e with (g,i) = k0, tile = k2, lane%16 = k1, c = k0 + 16*k1 + 256*k2.
+// With scale != 1 every result is also multiplied by scale, at no cost: the
+// factor rides on the start of the stage-1 recurrence, which multiplies
+// every point of the line once (in FP32, before the rounding to fp16).
 template <int SIGN>
-__device__ __forceinline__ void big_phase1(u32 (&d)[16][4], u32 (&e)[16][4], const BigTw& bt, const Dft16Op& op) {
+__device__ __forceinline__ void big_phase1(u32 (&d)[16][4], u32 (&e)[16][4], const BigTw& bt, const Dft16Op& op,
+                                           float scale = 1.f) {
     cf rec0[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) rec0[i] = raw_q<SIGN>(bt.rec0[i]);
+    for (int i = 0; i < 4; ++i) {
+        rec0[i] = raw_q<SIGN>(bt.rec0[i]);
+        rec0[i] = cf{rec0[i].x * scale, rec0[i].y * scale};
+    }
     const cf s2 = raw_q<SIGN>(bt.s2);
     auto make_op2 = [&] { return make_dft16_scaled(lane_id(), SIGN, s2); };
     dit4096<true>(d, SIGN, bt.tw, op, make_op2, rec0, [](int, int, cf v) -> cf { return v; });
@@ -301,12 +308,20 @@ __device__ __forceinline__ void big_phase2(u32 (&e)[16][4], int w, u32* lds, u32
                 u32 o[W][4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
-                    cf x[W];
+                    if constexpr (W == 2 && !SCALE) {
+                        // radix 2 on the packed words: one packed fp16 add
+                        // and subtract, each correctly rounded
+                        o[0][j] = hadd2(v[0][j], v[1][j]);
+                        o[1][j] = hsub2(v[0][j], v[1][j]);
+                    } else {
+                        cf x[W];
 #pragma unroll
-                    for (int s = 0; s < W; ++s) x[s] = unpack(v[s][j]);
-                    small_dft<W>(x, sign);
+                        for (int s = 0; s < W; ++s) x[s] = unpack(v[s][j]);
+                        small_dft<W>(x, sign);
 #pragma unroll
-                    for (int s = 0; s < W; ++s) o[s][j] = SCALE ? pack(x[s].x * scale, x[s].y * scale) : pack(x[s]);
+                        for (int s = 0; s < W; ++s)
+                            o[s][j] = SCALE ? pack(x[s].x * scale, x[s].y * scale) : pack(x[s]);
+                    }
                 }
                 const u32 c0 = 2048u * r + 4u * lane + 256u * (q + QN * w);
 #pragma unroll

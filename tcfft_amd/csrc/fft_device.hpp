@@ -49,6 +49,15 @@ __device__ __forceinline__ u32 pack(float re, float im) {
     return __builtin_bit_cast(u32, __builtin_convertvector(v, f16x2));
 }
 __device__ __forceinline__ u32 pack(cf v) { return pack(v.x, v.y); }
+// Packed fp16 complex sum / difference of two words (v_pk_add_f16): the
+// exact result rounded once to fp16, the same as or closer than an FP32
+// add followed by the rounding to fp16.
+__device__ __forceinline__ u32 hadd2(u32 a, u32 b) {
+    return __builtin_bit_cast(u32, __builtin_bit_cast(f16x2, a) + __builtin_bit_cast(f16x2, b));
+}
+__device__ __forceinline__ u32 hsub2(u32 a, u32 b) {
+    return __builtin_bit_cast(u32, __builtin_bit_cast(f16x2, a) - __builtin_bit_cast(f16x2, b));
+}
 __device__ __forceinline__ cf unpack(u32 w) {
     f16x2 h = __builtin_bit_cast(f16x2, w);
     return {(float)h[0], (float)h[1]};
