@@ -131,10 +131,34 @@ int tcfft_permute_to_dropin(tcfft_plan plan, const void *in, void *out, int n_li
 
 /* Split precision (no counterpart in the reference; config "FP16 hi+lo of
  * FP32 inputs"): FP32 complex input and output (interleaved float re/im,
- * natural order, in != out), every radix-16 stage as FP16 hi/lo products on
- * MFMA with FP32 accumulation, ~1e-7 relative error.  Currently n = 65536
- * plans only (else TCFFT_INVALID_SIZE). */
+ * natural order, the plan's 1D or 2D layout), every radix-16 stage as FP16
+ * hi/lo products on MFMA with FP32 accumulation, radix 2..8 steps in FP32;
+ * ~1e-7 relative error.  A plan prepared by tcfft_plan_prepare_split takes
+ * every size it was created for, in place or out of place; an unprepared
+ * plan only 1D n = 65536 out of place (else TCFFT_INVALID_SIZE, or
+ * TCFFT_INVALID_POINTER for in == out).  Partially overlapping buffers are
+ * rejected.  Real-input plans: TCFFT_INVALID_PLAN. */
 int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, void *stream);
+
+/* Prepares a complex plan for tcfft_exec_split at its size: builds the
+ * pass list (1D n = 65536 as two 256 x 256 passes; other shapes as passes
+ * of 16..256 rows, 2..4 per axis), uploads their twiddle tables and
+ * allocates a workspace of 8 bytes per element, so that tcfft_exec_split
+ * allocates nothing (graph-capturable) and runs in place.  One workspace per
+ * plan: split calls on one plan must not overlap.  Idempotent.  No
+ * counterpart in the reference. */
+int tcfft_plan_prepare_split(tcfft_plan plan);
+
+/* The split-precision pass list tcfft_plan_prepare_split builds for a shape
+ * (dims 1: n_or_nx = n, ny ignored; dims 2: nx x ny), computed on the host
+ * without a device.  Pass i fills desc[5i .. 5i+4] = {R, log2 Q, log2 Qlo,
+ * log2 M or 0, units}: each signal seen as R rows x Q columns, the R-point
+ * DFT down every column, output k of column q = q_hi*Qlo + q_lo times
+ * W_M^{q_hi k} (0: no twiddle) stored at q_hi*R*Qlo + k*Qlo + q_lo; units =
+ * 16-column blocks over the batch.  1D n = 65536 has no list (its own
+ * two-pass kernel): *n_passes = 0.  capacity < passes: TCFFT_INVALID_VALUE
+ * with *n_passes set.  No counterpart in the reference. */
+int tcfft_query_split(int dims, int n_or_nx, int ny, int n_batch, int *desc, int capacity, int *n_passes);
 
 /* Real-input transforms (no counterpart in the reference, whose callers feed
  * real data as complex with zero imaginary parts, e.g. the generator of

@@ -45,7 +45,7 @@ EXPORTS = (
     "tcfft_exec_dif", "tcfft_conv_exec", "tcfft_conv_filter_grad", "tcfft_plan_autotune",
     "tcfft_plan_create_1d_split", "tcfft_plan_create_1d_tuned", "tcfft_query_1d_split",
     "tcfft_plan_create_r2c_1d", "tcfft_exec_r2c", "tcfft_exec_c2r", "tcfft_permute_to_dropin",
-    "tcfft_exec_natural_noalloc", "tcfft_multi_exec_natural",
+    "tcfft_exec_natural_noalloc", "tcfft_multi_exec_natural", "tcfft_plan_prepare_split", "tcfft_query_split",
 )
 NOALLOC_SCRATCH_BYTES = 64 << 20  # TCFFT_NOALLOC_SCRATCH_BYTES
 OPT_SPLIT_CHUNK = 1
@@ -95,6 +95,9 @@ def lib():
         L.tcfft_plan_num_passes.argtypes = [_vp, _i32p]
         L.tcfft_exec_passes.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp]
         L.tcfft_exec_split.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+        L.tcfft_plan_prepare_split.argtypes = [_vp]
+        L.tcfft_query_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p, ctypes.c_int,
+                                        _i32p]
         L.tcfft_plan_set_option.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
         L.tcfft_plan_autotune.argtypes = [_vp, _vp, _vp, _i32p, _i32p, ctypes.POINTER(ctypes.c_float)]
         L.tcfft_plan_create_1d_split.argtypes = [ctypes.POINTER(_vp), ctypes.c_int, ctypes.c_int, ctypes.c_int]
@@ -167,6 +170,17 @@ def query_2d(nx: int, ny: int, batch: int = 1):
     return [int(v) for v in rx[: nrx.value]], [int(v) for v in ry[: nry.value]], npass.value
 
 
+def query_split(dims: int, n_or_nx: int, ny: int = 0, batch: int = 1):
+    """Split-precision pass list of a shape (tcfft_query_split): one dict per
+    pass with R, q_log2, qlo_log2, m_log2 (0 = no twiddle) and units."""
+    desc = np.zeros(5 * 16, dtype=np.int32)
+    npass = ctypes.c_int()
+    _check(lib().tcfft_query_split(dims, n_or_nx, ny, batch, desc.ctypes.data_as(_i32p), 16, ctypes.byref(npass)),
+           f"tcfft_query_split(dims={dims}, n_or_nx={n_or_nx}, ny={ny}, batch={batch})")
+    keys = ("R", "q_log2", "qlo_log2", "m_log2", "units")
+    return [dict(zip(keys, (int(v) for v in desc[5 * i: 5 * i + 5]))) for i in range(npass.value)]
+
+
 class _Plan:
     dims = 0
 
@@ -217,8 +231,15 @@ class _Plan:
                "tcfft_permute_to_dropin")
 
     def exec_split(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
-        """FP32 complex in/out (natural order, in != out), FP16 hi/lo MFMA split precision."""
+        """FP32 complex in/out (natural order), FP16 hi/lo MFMA split precision:
+        every size and in place after prepare_split(), else 1D n = 65536 out
+        of place only."""
         _check(lib().tcfft_exec_split(self._h, _vp(in_ptr), _vp(out_ptr), direction, _vp(stream)), "tcfft_exec_split")
+
+    def prepare_split(self):
+        """Pass list, twiddle tables and the 8-byte-per-element workspace of
+        exec_split at this plan's size (allocated here, not in exec)."""
+        _check(lib().tcfft_plan_prepare_split(self._h), "tcfft_plan_prepare_split")
 
     def exec_dif(self, in_ptr: int, out_ptr: int, direction: int = FORWARD, stream: int = 0):
         """Natural-order input -> spectrum in the digit-reversed order of

@@ -47,12 +47,31 @@ hipError_t launch_big(int W, uint32_t* buf, uint32_t n_signals, int sign, const 
 hipError_t launch_cols_big(int W, uint32_t* buf, uint32_t s_log2, uint32_t outer, int sign, const float2* w4096,
                            StartSched sched, hipStream_t st);
 
-// Split-precision N = 65536 transform of FP32 complex data (in != out), as
-// two passes per chunk of `chunk` signals (0 = all at once); `sched` = the
+// Split-precision N = 65536 transform of FP32 complex data, as two passes per
+// chunk of `chunk` signals (0 = all at once): in -> mid (transposed), then
+// mid -> out (mid may be out; in must differ from mid).  `sched` = the
 // passes' start schedule (delay and groups; the resident count is filled in).
-hipError_t launch_split_65536(const float2* in, float2* out, uint32_t n_signals, uint32_t chunk, int sign,
-                              const float2* w4096, const float2* wn_hi, const float2* wn_lo, StartSched sched,
-                              hipStream_t st);
+hipError_t launch_split_65536(const float2* in, float2* mid, float2* out, uint32_t n_signals, uint32_t chunk,
+                              int sign, const float2* w4096, const float2* wn_hi, const float2* wn_lo,
+                              StartSched sched, hipStream_t st);
+
+// One split-precision column pass (split.hip): each signal is R rows x Q
+// columns (element (r, q) at r*Q + q); the R-point DFT down every column,
+// output k of column q = q_hi*Qlo + q_lo times W_M^{q_hi k} (M = R*Q/Qlo;
+// m_mask = 0: no twiddle), stored at q_hi*R*Qlo + k*Qlo + q_lo.  Qlo = 1 is
+// the row store (row = true); otherwise Qlo >= 16.  R = 16..256 on MFMA,
+// R = 2, 4, 8 (no twiddle, Qlo = Q) on VALU.  src != dst unless Qlo = Q.
+struct SplitPassArgs {
+    uint32_t q_log2;    // Q = 2^q_log2 >= 16
+    uint32_t qlo_log2;  // Qlo = 2^qlo_log2
+    uint32_t units;     // 16-column units: signals * Q / 16
+    uint32_t m_mask;    // M - 1, 0 = no twiddle
+    uint32_t lo_bits;   // W_M^m = hi[m >> lo_bits] * lo[m & (2^lo_bits - 1)]
+    const float2* hi;
+    const float2* lo;
+};
+hipError_t launch_split_pass(int R, bool row, const float2* src, float2* dst, const SplitPassArgs& a, int sign,
+                             const float2* w4096, StartSched sched, hipStream_t st);
 
 // Uneven batch: one task = signals [first, first + count) of one size class.
 struct MultiTask {

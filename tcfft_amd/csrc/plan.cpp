@@ -219,6 +219,18 @@ struct tcfft_plan_s {
     int rows_waves = 1;
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
     int split_delay = 2;  // start delay of the split-precision passes (3 groups; 2^16 x 256: 110.6 -> 106.6 us)
+    // Split precision at every size (tcfft_plan_prepare_split): the column
+    // passes in order, and a workspace of total * 8 bytes (FP32 complex)
+    // for the passes whose transposed store cannot go in place.
+    struct SplitStep {
+        int R;
+        bool row;          // Qlo = 1 (transposed row store)
+        bool transposing;  // not the last pass of its axis: src != dst
+        tcfft::SplitPassArgs a;
+    };
+    std::vector<SplitStep> split_steps;
+    bool split_ready = false;
+    float2* split_ws = nullptr;
     int real_n = 0;       // real-input plan: the real length (its complex layout has n = real_n / 2)
     float2* w_real = nullptr;  // W_{real_n}^0..64 (n <= 8192) or ^0..64W (n = 8192 W <= 65536)
     std::pair<float2*, float2*> w_real_pair{};  // n > 65536: two-level table of W_n
@@ -287,6 +299,56 @@ int table_pair(tcfft_plan_s* p, uint32_t M, std::pair<float2*, float2*>& out) {
     }
     out = it->second;
     return TCFFT_SUCCESS;
+}
+
+// Split-precision passes for the M-point DFTs along the first index of
+// `signals` arrays [M][L] (M = 2^m_log2, L = 2^l_log2): one pass of radix M
+// when L >= 16 and M <= 256 (radix 2..8 on VALU), else ceil(m_log2 / 8)
+// passes (at least two when L = 1, whose first pass needs >= 16 columns) of
+// 4..8 bits, balanced.  Pass i sees R_i rows by Q = M*L/R_i columns with
+// Qlo = L * R_0 * ... * R_{i-1}, and all but the last take the twiddle of
+// the remaining M' = M / (R_0 ... R_{i-1}) points.
+int split_axis(int m_log2, int l_log2, uint32_t signals, std::vector<tcfft_plan_s::SplitStep>& out) {
+    std::vector<int> bits;
+    if (m_log2 <= 3) {
+        if (l_log2 < 4 || m_log2 < 1) return TCFFT_INVALID_SIZE;
+        bits.push_back(m_log2);
+    } else {
+        int passes = (m_log2 + 7) / 8;
+        if (l_log2 == 0 && passes < 2) passes = 2;
+        for (int i = 0; i < passes; ++i) bits.push_back((m_log2 + i) / passes);
+        for (int b : bits)
+            if (b < 4 || b > 8) return TCFFT_INVALID_SIZE;
+    }
+    int done = 0;
+    for (size_t i = 0; i < bits.size(); ++i) {
+        const int b = bits[i];
+        const int cur_l = l_log2 + done, cur_m = m_log2 - done;
+        tcfft_plan_s::SplitStep s{};
+        s.R = 1 << b;
+        s.row = cur_l == 0;
+        s.transposing = cur_m > b;
+        s.a.q_log2 = (uint32_t)(m_log2 + l_log2 - b);
+        s.a.qlo_log2 = (uint32_t)cur_l;
+        s.a.units = signals << (s.a.q_log2 - 4);
+        if (s.transposing) {
+            s.a.m_mask = (1u << cur_m) - 1u;
+            s.a.lo_bits = (uint32_t)cur_m / 2u;  // the split of table_pair
+        }
+        out.push_back(s);
+        done += b;
+    }
+    return TCFFT_SUCCESS;
+}
+
+// The split-precision pass list of a shape (host only): 1D n (ny = 0) or 2D
+// nx x ny, batch signals; n = 65536 has none (its own two-pass kernel).
+int split_steps_of(int dims, int n_or_nx, int ny, int batch, std::vector<tcfft_plan_s::SplitStep>& steps) {
+    if (dims == 1) return n_or_nx == 65536 ? TCFFT_SUCCESS : split_axis(ilog2((uint32_t)n_or_nx), 0, (uint32_t)batch, steps);
+    // rows: the contiguous ny axis of every one of the nx lines; then the nx
+    // axis with the ny columns of each image
+    const int rc = split_axis(ilog2((uint32_t)ny), 0, (uint32_t)batch * (uint32_t)n_or_nx, steps);
+    return rc ? rc : split_axis(ilog2((uint32_t)n_or_nx), ilog2((uint32_t)ny), (uint32_t)batch, steps);
 }
 
 int finish_plan(tcfft_plan_s* p) {
@@ -609,6 +671,29 @@ int tcfft_query_2d(int nx, int ny, int n_batch, int* radices_x, int* n_radices_x
     return copy_radices(lay.rad_y, radices_y, capacity, n_radices_y);
 }
 
+int tcfft_query_split(int dims, int n_or_nx, int ny, int n_batch, int* desc, int capacity, int* n_passes) {
+    if (!n_passes || (capacity > 0 && !desc)) return TCFFT_INVALID_POINTER;
+    Layout lay;
+    int rc = dims == 1 ? build_layout_1d(n_or_nx, n_batch, lay)
+                       : dims == 2 ? build_layout_2d(n_or_nx, ny, n_batch, lay) : TCFFT_INVALID_VALUE;
+    if (rc) return rc;
+    std::vector<tcfft_plan_s::SplitStep> steps;
+    rc = split_steps_of(dims, n_or_nx, ny, n_batch, steps);
+    if (rc) return rc;
+    *n_passes = (int)steps.size();
+    if ((int)steps.size() > capacity) return TCFFT_INVALID_VALUE;
+    for (size_t i = 0; i < steps.size(); ++i) {
+        const auto& s = steps[i];
+        int* d = desc + 5 * i;
+        d[0] = s.R;
+        d[1] = (int)s.a.q_log2;
+        d[2] = (int)s.a.qlo_log2;
+        d[3] = s.a.m_mask ? ilog2(s.a.m_mask + 1u) : 0;
+        d[4] = (int)s.a.units;
+    }
+    return TCFFT_SUCCESS;
+}
+
 int tcfft_exec(tcfft_plan plan, void* data, int direction, void* stream) {
     if (!valid(plan)) return TCFFT_INVALID_PLAN;
     if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
@@ -722,23 +807,97 @@ int tcfft_exec_c2r(tcfft_plan plan, const void* in, void* out, void* stream) {
     return TCFFT_SUCCESS;
 }
 
+int tcfft_plan_prepare_split(tcfft_plan plan) {
+    if (!valid(plan)) return TCFFT_INVALID_PLAN;
+    if (plan->real_n) return TCFFT_INVALID_PLAN;
+    if (plan->split_ready) return TCFFT_SUCCESS;
+    DeviceGuard guard(plan->device);
+    const Layout& L = plan->lay;
+    std::vector<tcfft_plan_s::SplitStep> steps;
+    int rc = split_steps_of(L.dims, L.dims == 1 ? L.n : L.nx, L.ny, L.batch, steps);
+    if (rc) return rc;
+    if (L.dims == 1 && L.n == 65536) {
+        std::pair<float2*, float2*> t;
+        rc = table_pair(plan, 16, t);
+        if (rc) return rc;
+    }
+    for (auto& s : steps) {
+        if (!s.a.m_mask) continue;
+        std::pair<float2*, float2*> t;
+        rc = table_pair(plan, (uint32_t)ilog2(s.a.m_mask + 1u), t);
+        if (rc) return rc;
+        s.a.hi = t.first;
+        s.a.lo = t.second;
+    }
+    void* ws = nullptr;
+    const hipError_t e = hipMalloc(&ws, plan->total * 8u);
+    if (e != hipSuccess) return hip_fail(e);
+    plan->allocs.push_back(ws);
+    plan->split_ws = (float2*)ws;
+    plan->split_steps = std::move(steps);
+    plan->split_ready = true;
+    return TCFFT_SUCCESS;
+}
+
 int tcfft_exec_split(tcfft_plan plan, const void* in, void* out, int direction, void* stream) {
     if (!valid(plan)) return TCFFT_INVALID_PLAN;
-    if (!in || !out || in == out || ((uintptr_t)in & 15u) || ((uintptr_t)out & 15u)) return TCFFT_INVALID_POINTER;
+    if (plan->real_n) return TCFFT_INVALID_PLAN;
+    if (!in || !out || ((uintptr_t)in & 15u) || ((uintptr_t)out & 15u)) return TCFFT_INVALID_POINTER;
     if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
     const Layout& L = plan->lay;
-    if (L.dims != 1 || L.n != 65536) return TCFFT_INVALID_SIZE;
+    const size_t bytes = plan->total * 8u;
+    const uintptr_t a = (uintptr_t)in, b = (uintptr_t)out;
+    if (in != out && a < b + bytes && b < a + bytes) return TCFFT_INVALID_POINTER;  // partial overlap
+    const bool is65536 = L.dims == 1 && L.n == 65536;
+    if (!plan->split_ready) {
+        // unprepared plans: the 65536-point transform out of place only
+        if (!is65536) return TCFFT_INVALID_SIZE;
+        if (in == out) return TCFFT_INVALID_POINTER;
+    }
     DeviceGuard guard(plan->device);
-    std::pair<float2*, float2*> t;
-    int rc = table_pair(plan, 16, t);
-    if (rc) return rc;
     const tcfft::StartSched sched{(uint32_t)plan->split_delay, 3u, 0u};
     hipStream_t st = (hipStream_t)stream;
-    const uint32_t batch = (uint32_t)L.batch;
-    const uint32_t chunk = plan->split_chunk <= 0 || plan->split_chunk > (int)batch ? batch : (uint32_t)plan->split_chunk;
-    const hipError_t e = tcfft::launch_split_65536((const float2*)in, (float2*)out, batch, chunk, direction,
-                                                   plan->w4096, t.first, t.second, sched, st);
-    return e == hipSuccess ? TCFFT_SUCCESS : hip_fail(e);
+    if (is65536) {
+        std::pair<float2*, float2*> t;
+        int rc = table_pair(plan, 16, t);
+        if (rc) return rc;
+        const uint32_t batch = (uint32_t)L.batch;
+        const uint32_t chunk =
+            plan->split_chunk <= 0 || plan->split_chunk > (int)batch ? batch : (uint32_t)plan->split_chunk;
+        // out of place: the intermediate in out; in place: in the workspace
+        float2* mid = in == out ? plan->split_ws : (float2*)out;
+        const hipError_t e = tcfft::launch_split_65536((const float2*)in, mid, (float2*)out, batch, chunk, direction,
+                                                       plan->w4096, t.first, t.second, sched, st);
+        return e == hipSuccess ? TCFFT_SUCCESS : hip_fail(e);
+    }
+    // Buffers, from the last pass back: it writes out; a transposing pass
+    // reads the other buffer than it writes, a last pass of an axis the same
+    // one.  The first pass reads in; if that is where it would write (in
+    // place), the last pass reads the workspace instead, which flips every
+    // earlier choice.
+    const auto& S = plan->split_steps;
+    const int P = (int)S.size();
+    std::vector<float2*> dst(P);
+    float2* const OUT = (float2*)out;
+    float2* const WS = plan->split_ws;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        float2* cur_dst = OUT;
+        float2* src_last = attempt ? WS : OUT;
+        for (int j = P - 1; j >= 0; --j) {
+            dst[j] = cur_dst;
+            const float2* src = j == P - 1 ? src_last : (S[j].transposing ? (cur_dst == OUT ? WS : OUT) : cur_dst);
+            cur_dst = (float2*)src;
+        }
+        if (!(S[0].transposing && dst[0] == (const float2*)in)) break;
+    }
+    const float2* src = (const float2*)in;
+    for (int j = 0; j < P; ++j) {
+        const hipError_t e =
+            tcfft::launch_split_pass(S[j].R, S[j].row, src, dst[j], S[j].a, direction, plan->w4096, sched, st);
+        if (e != hipSuccess) return hip_fail(e);
+        src = dst[j];
+    }
+    return TCFFT_SUCCESS;
 }
 
 }  // extern "C"

@@ -290,8 +290,9 @@ static u32 split_resident(K k) {
 }
 
 template <int SIGN>
-static hipError_t launch_split_t(const float2* in, float2* out, u32 n_signals, u32 chunk, const float2* w4096,
-                                 const float2* wn_hi, const float2* wn_lo, StartSched sched, hipStream_t st) {
+static hipError_t launch_split_t(const float2* in, float2* mid, float2* out, u32 n_signals, u32 chunk,
+                                 const float2* w4096, const float2* wn_hi, const float2* wn_lo, StartSched sched,
+                                 hipStream_t st) {
     // one cached count per kernel (each instantiation of this function and
     // each variable below is a distinct static)
     static const u32 res1 = split_resident(k_split_col<1, SIGN>);
@@ -304,18 +305,419 @@ static hipError_t launch_split_t(const float2* in, float2* out, u32 n_signals, u
     for (u32 s0 = 0; s0 < n_signals; s0 += chunk) {
         const u32 cnt = min(chunk, n_signals - s0);
         const size_t off = (size_t)s0 * 65536;
-        hipLaunchKernelGGL((k_split_col<1, SIGN>), dim3(cnt * 4), dim3(256), 0, st, in + off, out + off, w4096, wn_hi,
+        hipLaunchKernelGGL((k_split_col<1, SIGN>), dim3(cnt * 4), dim3(256), 0, st, in + off, mid + off, w4096, wn_hi,
                            wn_lo, s1);
-        hipLaunchKernelGGL((k_split_col<2, SIGN>), dim3(cnt * 4), dim3(256), 0, st, out + off, out + off, w4096, wn_hi,
+        hipLaunchKernelGGL((k_split_col<2, SIGN>), dim3(cnt * 4), dim3(256), 0, st, mid + off, out + off, w4096, wn_hi,
                            wn_lo, s2);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_split_65536(const float2* in, float2* out, u32 n_signals, u32 chunk, int sign, const float2* w4096,
-                              const float2* wn_hi, const float2* wn_lo, StartSched sched, hipStream_t st) {
-    return sign < 0 ? launch_split_t<-1>(in, out, n_signals, chunk, w4096, wn_hi, wn_lo, sched, st)
-                    : launch_split_t<+1>(in, out, n_signals, chunk, w4096, wn_hi, wn_lo, sched, st);
+hipError_t launch_split_65536(const float2* in, float2* mid, float2* out, u32 n_signals, u32 chunk, int sign,
+                              const float2* w4096, const float2* wn_hi, const float2* wn_lo, StartSched sched,
+                              hipStream_t st) {
+    return sign < 0 ? launch_split_t<-1>(in, mid, out, n_signals, chunk, w4096, wn_hi, wn_lo, sched, st)
+                    : launch_split_t<+1>(in, mid, out, n_signals, chunk, w4096, wn_hi, wn_lo, sched, st);
+}
+
+// ---------------------------------------------------------------------------
+// Every other shape: a sequence of column passes (SplitPassArgs).  A pass
+// sees each signal as R rows x Q columns, element (r, q) at r*Q + q, and
+// computes the R-point DFT down every column.  With q = q_hi*Qlo + q_lo it
+// multiplies output k by W_M^{q_hi k} (M = R*Q/Qlo; no twiddle in the last
+// pass of an axis) and stores it at q_hi*R*Qlo + k*Qlo + q_lo.  That is one
+// step of the four-step recursion for the M-point DFTs of an [M][L] array
+// along its first index (Qlo = L): x[c + (M/R) r][l] -> U[c][k][l] holds the
+// (M/R)-point problems of U along c with R*L columns, so every pass again
+// reads columns, and the last one (Q = Qlo) leaves natural order in place.
+//
+// A unit is 16 adjacent columns of one signal; a wave holds 4096 points:
+// 256/R units of R rows (tile t = unit*R/16 + r0).  R = 16*F (F = 1..8):
+//   stage 1 (tile r0, rows r = r0 + F r1, r1 = 4g + j): split DFT-16 over r1
+//   -> rows k0 = 4g + i; twiddle W_R^{r0 k0}; DFT-F over the F tiles of the
+//   unit in FP32 on VALU -> k = k0 + 16 k1 with k1 = the tile.
+// R = 256: the two split MFMA stages of the 65536-point kernel above.
+// Each unit takes its own power-of-two scale (wave_scale over its 16 columns),
+// so units of different signals do not share one.
+
+__device__ __forceinline__ cf split_tw(const SplitPassArgs& a, u32 e, float sg) {
+    e &= a.m_mask;
+    const float2 h = a.hi[e >> a.lo_bits], l = a.lo[e & ((1u << a.lo_bits) - 1u)];
+    return cmul(cf{h.x, sg * h.y}, cf{l.x, sg * l.y});
+}
+
+__device__ __forceinline__ float uniform_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
+template <int R, bool ROW, int SIGN>
+__device__ __forceinline__ void split_pass_f(const float2* src, float2* dst, const SplitPassArgs& a,
+                                             const float2* __restrict__ w4096) {
+    constexpr int F = R / 16, G = 16 / F, LR = F == 1 ? 4 : F == 2 ? 5 : F == 4 ? 6 : 7;
+    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
+    const int lane = lane_id(), c16 = lane & 15, g = lane >> 4;
+    const u32 wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const u32 upq = a.q_log2 - 4u;  // log2 of units per signal
+    // the stage-1 twiddle steps W_R^{4g+i} first (global loads complete in
+    // issue order: a table value loaded after the data would wait for it)
+    float2 t_step[4];
+    if constexpr (F > 1) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) t_step[i] = w4096[(4096 / R) * (4 * g + i)];
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float dr[16][4], di[16][4];
+#pragma unroll
+    for (int cg = 0; cg < G; ++cg) {
+        const u32 u = min(wave * G + cg, a.units - 1u);  // a ragged wave re-reads the last unit
+        const float2* p = src + ((size_t)(u >> upq) << (a.q_log2 + LR)) + ((u & ((1u << upq) - 1u)) << 4) + c16 +
+                          ((size_t)(4 * F * g) << a.q_log2);
+#pragma unroll
+        for (int r0 = 0; r0 < F; ++r0)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const f32x2 v = __builtin_nontemporal_load(
+                    reinterpret_cast<const f32x2*>(p + ((size_t)(r0 + F * j) << a.q_log2)));
+                dr[cg * F + r0][j] = v[0];
+                di[cg * F + r0][j] = v[1];
+            }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const Dft16Split Fop = make_dft16_split(lane, SIGN);
+    float inv_u[G];
+#pragma unroll
+    for (int cg = 0; cg < G; ++cg) {
+        float m = 0.f;
+#pragma unroll
+        for (int r0 = 0; r0 < F; ++r0)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) m = fmaxf(m, fmaxf(fabsf(dr[cg * F + r0][j]), fabsf(di[cg * F + r0][j])));
+        float s, inv;
+        wave_scale(m, s, inv);
+        s = uniform_f(s);
+        inv_u[cg] = uniform_f(inv);
+#pragma unroll
+        for (int r0 = 0; r0 < F; ++r0) {
+            const int t = cg * F + r0;
+            f32x4 ore, oim;
+            dft16_split_b<true>(dr[t], di[t], s, Fop, ore, oim);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                dr[t][i] = ore[i];
+                di[t][i] = oim[i];
+            }
+        }
+    }
+    if constexpr (F > 1) {
+#pragma unroll
+        for (int cg = 0; cg < G; ++cg)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const cf step = {t_step[i].x, sg * t_step[i].y};
+                cf v[F];
+                v[0] = cf{dr[cg * F][i], di[cg * F][i]};
+                cf w = step;
+#pragma unroll
+                for (int r0 = 1; r0 < F; ++r0) {
+                    v[r0] = cmul(cf{dr[cg * F + r0][i], di[cg * F + r0][i]}, w);
+                    if (r0 + 1 < F) w = cmul(w, step);
+                }
+                small_dft<F>(v, SIGN);
+#pragma unroll
+                for (int k1 = 0; k1 < F; ++k1) {
+                    dr[cg * F + k1][i] = v[k1].x;
+                    di[cg * F + k1][i] = v[k1].y;
+                }
+            }
+    }
+    // outputs: tile cg*F + k1, word i -> k = 4g + i + 16 k1 of unit cg
+#pragma unroll
+    for (int cg = 0; cg < G; ++cg) {
+        const u32 u = wave * G + cg;
+        if (u >= a.units) break;  // wave-uniform
+        const u32 q = ((u & ((1u << upq) - 1u)) << 4) + c16;
+        float2* base = dst + ((size_t)(u >> upq) << (a.q_log2 + LR));
+        const float inv = inv_u[cg];
+        if constexpr (ROW) {
+            // W_M^{q k}, M = R*Q, as W^{q(4g+i)} (W^{16q})^{k1}; row q of R
+            // outputs, four consecutive k per lane (two 16-byte stores)
+            cf fa[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const cf v = split_tw(a, q * (4u * g + i), sg);
+                fa[i] = cf{v.x * inv, v.y * inv};
+            }
+            const cf fstep = split_tw(a, 16u * q, sg);
+            cf fb = {1.f, 0.f};
+            float2* o = base + ((size_t)q << LR) + 4 * g;
+#pragma unroll
+            for (int k1 = 0; k1 < F; ++k1) {
+                float ov[8];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const cf v = cmul(cf{dr[cg * F + k1][i], di[cg * F + k1][i]}, k1 ? cmul(fa[i], fb) : fa[i]);
+                    ov[2 * i] = v.x;
+                    ov[2 * i + 1] = v.y;
+                }
+                if (k1 + 1 < F) fb = k1 ? cmul(fb, fstep) : fstep;
+                const f32x4 o0 = {ov[0], ov[1], ov[2], ov[3]}, o1 = {ov[4], ov[5], ov[6], ov[7]};
+                *reinterpret_cast<f32x4*>(o + 16 * k1) = o0;
+                *reinterpret_cast<f32x4*>(o + 16 * k1 + 2) = o1;
+            }
+        } else {
+            const u32 qhi = q >> a.qlo_log2, qlo = q & ((1u << a.qlo_log2) - 1u);
+            float2* o = base + ((size_t)qhi << (a.qlo_log2 + LR)) + qlo;
+            cf fa[4], fstep = {1.f, 0.f};
+            const bool tw = a.m_mask != 0u;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const cf v = tw ? split_tw(a, qhi * (4u * g + i), sg) : cf{1.f, 0.f};
+                fa[i] = cf{v.x * inv, v.y * inv};
+            }
+            if (tw) fstep = split_tw(a, 16u * qhi, sg);
+            cf fb = {1.f, 0.f};
+#pragma unroll
+            for (int k1 = 0; k1 < F; ++k1) {
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const cf v = cmul(cf{dr[cg * F + k1][i], di[cg * F + k1][i]}, k1 ? cmul(fa[i], fb) : fa[i]);
+                    const f32x2 w2 = {v.x, v.y};
+                    __builtin_nontemporal_store(
+                        w2, reinterpret_cast<f32x2*>(o + ((size_t)(4 * g + i + 16 * k1) << a.qlo_log2)));
+                }
+                if (k1 + 1 < F) fb = k1 ? cmul(fb, fstep) : fstep;
+            }
+        }
+    }
+}
+
+template <bool ROW, int SIGN>
+__device__ __forceinline__ void split_pass_256(const float2* src, float2* dst, const SplitPassArgs& a,
+                                               const float2* __restrict__ w4096) {
+    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
+    const int lane = lane_id(), c16 = lane & 15, g = lane >> 4;
+    const u32 wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const u32 upq = a.q_log2 - 4u;
+    const u32 u = min(wave, a.units - 1u);
+    const u32 q = ((u & ((1u << upq) - 1u)) << 4) + c16;
+    const size_t sbase = (size_t)(u >> upq) << (a.q_log2 + 8);
+
+    float2 t_step[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) t_step[i] = w4096[16 * (4 * g + i)];
+    __builtin_amdgcn_sched_barrier(0);
+    // rows r = r0 + 16 (4g + j): tile r0, word j
+    float dr[16][4], di[16][4];
+    {
+        const float2* p = src + sbase + q + ((size_t)(64 * g) << a.q_log2);
+#pragma unroll
+        for (int r0 = 0; r0 < 16; ++r0)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const f32x2 v = __builtin_nontemporal_load(
+                    reinterpret_cast<const f32x2*>(p + ((size_t)(r0 + 16 * j) << a.q_log2)));
+                dr[r0][j] = v[0];
+                di[r0][j] = v[1];
+            }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    float m = 0.f;
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) m = fmaxf(m, fmaxf(fabsf(dr[t][j]), fabsf(di[t][j])));
+    float s, inv;
+    wave_scale(m, s, inv);
+    {
+        const Dft16Split F = make_dft16_split(lane, SIGN);
+        cf step[4], rec[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            step[i] = cf{t_step[i].x, sg * t_step[i].y};
+            rec[i] = cf{0.0625f, 0.f};
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            f32x4 ore, oim;
+            dft16_split_b<true>(dr[t], di[t], s, F, ore, oim);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const cf v = cmul(cf{ore[i], oim[i]}, rec[i]);
+                if (t < 15) rec[i] = cmul(rec[i], step[i]);
+                dr[t][i] = v.x;
+                di[t][i] = v.y;
+            }
+        }
+    }
+    {
+        float tr[16][4], ti[16][4];
+        transpose_gi_tile(dr, tr);
+        transpose_gi_tile(di, ti);
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                dr[t][i] = tr[t][i];
+                di[t][i] = ti[t][i];
+            }
+    }
+    {
+        // the DFT-16 operand again, from the opaque lane id (kept live from
+        // stage 1 it would take registers through the transpose)
+        const Dft16Split F2 = make_dft16_split(lane_id(), SIGN);
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            f32x4 ore, oim;
+            dft16_split_b<false>(dr[t], di[t], 1.f, F2, ore, oim);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                dr[t][i] = ore[i];
+                di[t][i] = oim[i];
+            }
+        }
+    }
+    if (wave >= a.units) return;  // wave-uniform; nothing to store
+    // outputs: tile t = k0, (g, i) = k1 -> k = t + 16 (4g + i); inv2 undoes
+    // the scale and the 1/16
+    const float inv2 = inv * 16.f;
+    if constexpr (ROW) {
+        // -> (g, i) = k0, tile = k1: four consecutive k per lane, row q of 256
+        float ta[16][4], tb[16][4];
+        transpose_gi_tile(dr, ta);
+        transpose_gi_tile(di, tb);
+        cf fa[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const cf v = split_tw(a, q * (4u * g + i), sg);
+            fa[i] = cf{v.x * inv2, v.y * inv2};
+        }
+        const cf fstep = split_tw(a, 16u * q, sg);
+        cf fb = {1.f, 0.f};
+        float2* o = dst + sbase + ((size_t)q << 8) + 4 * g;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            float ov[8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const cf v = cmul(cf{ta[t][i], tb[t][i]}, t ? cmul(fa[i], fb) : fa[i]);
+                ov[2 * i] = v.x;
+                ov[2 * i + 1] = v.y;
+            }
+            if (t < 15) fb = t ? cmul(fb, fstep) : fstep;
+            const f32x4 o0 = {ov[0], ov[1], ov[2], ov[3]}, o1 = {ov[4], ov[5], ov[6], ov[7]};
+            *reinterpret_cast<f32x4*>(o + 16 * t) = o0;
+            *reinterpret_cast<f32x4*>(o + 16 * t + 2) = o1;
+        }
+    } else {
+        const u32 qhi = q >> a.qlo_log2, qlo = q & ((1u << a.qlo_log2) - 1u);
+        float2* o = dst + sbase + ((size_t)qhi << (a.qlo_log2 + 8)) + qlo;
+        // W_M^{q_hi k} = W^{16 q_hi (4g+i)} (W^{q_hi})^t
+        cf fa[4], fstep = {1.f, 0.f};
+        const bool tw = a.m_mask != 0u;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const cf v = tw ? split_tw(a, 16u * qhi * (4u * g + i), sg) : cf{1.f, 0.f};
+            fa[i] = cf{v.x * inv2, v.y * inv2};
+        }
+        if (tw) fstep = split_tw(a, qhi, sg);
+        cf fb = {1.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const cf v = cmul(cf{dr[t][i], di[t][i]}, t ? cmul(fa[i], fb) : fa[i]);
+                const f32x2 w2 = {v.x, v.y};
+                __builtin_nontemporal_store(w2,
+                                            reinterpret_cast<f32x2*>(o + ((size_t)(t + 64 * g + 16 * i) << a.qlo_log2)));
+            }
+            if (t < 15) fb = t ? cmul(fb, fstep) : fstep;
+        }
+    }
+}
+
+template <int R, bool ROW, int SIGN>
+__global__ __launch_bounds__(256, 2) void k_split_pass(const float2* src, float2* dst, SplitPassArgs a,
+                                                    const float2* __restrict__ w4096, StartSched sched) {
+    split_start_delay(sched);
+    if constexpr (R == 256)
+        split_pass_256<ROW, SIGN>(src, dst, a, w4096);
+    else
+        split_pass_f<R, ROW, SIGN>(src, dst, a, w4096);
+}
+
+// Radix R = 2, 4, 8 alone (the strided axis of a 2D shape with nx <= 8): one
+// thread per column, FP32 on VALU (the only products are by +-1, +-i and
+// (+-1 +-i)/sqrt(2)), no twiddle; in place or out of place.
+template <int R, int SIGN>
+__global__ __launch_bounds__(256) void k_split_small(const float2* src, float2* dst, u32 q_log2, u32 cols) {
+    constexpr int LR = R == 2 ? 1 : R == 4 ? 2 : 3;
+    const u32 id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= cols) return;
+    const size_t base = ((size_t)(id >> q_log2) << (q_log2 + LR)) + (id & ((1u << q_log2) - 1u));
+    cf v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const f32x2 x = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(src + base + ((size_t)r << q_log2)));
+        v[r] = cf{x[0], x[1]};
+    }
+    small_dft<R>(v, SIGN);
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+        const f32x2 x = {v[k].x, v[k].y};
+        __builtin_nontemporal_store(x, reinterpret_cast<f32x2*>(dst + base + ((size_t)k << q_log2)));
+    }
+}
+
+template <int R, bool ROW, int SIGN>
+static hipError_t launch_pass_t(const float2* src, float2* dst, const SplitPassArgs& a, const float2* w4096,
+                                StartSched sched, hipStream_t st) {
+    static const u32 res = split_resident(k_split_pass<R, ROW, SIGN>);
+    constexpr u32 G = 256 / R;  // units per wave
+    const u32 waves = (a.units + G - 1) / G;
+    if (sched.groups == 0) sched.delay = 0;
+    sched.resident = res;
+    hipLaunchKernelGGL((k_split_pass<R, ROW, SIGN>), dim3((waves + 3) / 4), dim3(256), 0, st, src, dst, a, w4096,
+                       sched);
+    return hipGetLastError();
+}
+
+template <int R, int SIGN>
+static hipError_t launch_small_t(const float2* src, float2* dst, const SplitPassArgs& a, hipStream_t st) {
+    const u32 cols = a.units * 16u;
+    hipLaunchKernelGGL((k_split_small<R, SIGN>), dim3((cols + 255) / 256), dim3(256), 0, st, src, dst, a.q_log2, cols);
+    return hipGetLastError();
+}
+
+template <int SIGN>
+static hipError_t launch_pass_s(int R, bool row, const float2* src, float2* dst, const SplitPassArgs& a,
+                                const float2* w4096, StartSched sched, hipStream_t st) {
+    switch (R) {
+    case 2: return row ? hipErrorInvalidValue : launch_small_t<2, SIGN>(src, dst, a, st);
+    case 4: return row ? hipErrorInvalidValue : launch_small_t<4, SIGN>(src, dst, a, st);
+    case 8: return row ? hipErrorInvalidValue : launch_small_t<8, SIGN>(src, dst, a, st);
+    case 16:
+        return row ? launch_pass_t<16, true, SIGN>(src, dst, a, w4096, sched, st)
+                   : launch_pass_t<16, false, SIGN>(src, dst, a, w4096, sched, st);
+    case 32:
+        return row ? launch_pass_t<32, true, SIGN>(src, dst, a, w4096, sched, st)
+                   : launch_pass_t<32, false, SIGN>(src, dst, a, w4096, sched, st);
+    case 64:
+        return row ? launch_pass_t<64, true, SIGN>(src, dst, a, w4096, sched, st)
+                   : launch_pass_t<64, false, SIGN>(src, dst, a, w4096, sched, st);
+    case 128:
+        return row ? launch_pass_t<128, true, SIGN>(src, dst, a, w4096, sched, st)
+                   : launch_pass_t<128, false, SIGN>(src, dst, a, w4096, sched, st);
+    case 256:
+        return row ? launch_pass_t<256, true, SIGN>(src, dst, a, w4096, sched, st)
+                   : launch_pass_t<256, false, SIGN>(src, dst, a, w4096, sched, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_split_pass(int R, bool row, const float2* src, float2* dst, const SplitPassArgs& a, int sign,
+                             const float2* w4096, StartSched sched, hipStream_t st) {
+    return sign < 0 ? launch_pass_s<-1>(R, row, src, dst, a, w4096, sched, st)
+                    : launch_pass_s<+1>(R, row, src, dst, a, w4096, sched, st);
 }
 
 }  // namespace tcfft
