@@ -378,6 +378,38 @@ def bench_split(torch, tcfft_amd, reps, chunk=None, ms_plain=None):
     return out
 
 
+def bench_split_shapes(torch, tcfft_amd, reps):
+    """Split precision on prepared plans at the other configured shapes
+    (1D 2^20 x 16, 2D 1024^2 x 8), FP32 in/out out of place, rotated
+    buffers; rel-L2 of the first signal / image against FP64."""
+    res = {}
+    for name, dims, a, b, batch in (("split_n2^20_b16", 1, 1 << 20, 0, 16),
+                                    ("split_2d_1024x1024_b8", 2, 1024, 1024, 8)):
+        plan = tcfft_amd.Plan1D(a, batch) if dims == 1 else tcfft_amd.Plan2D(a, b, batch)
+        plan.prepare_split()
+        per = a * (b or 1)
+        rng = np.random.default_rng(4)
+        x = (rng.standard_normal((batch, per)) + 1j * rng.standard_normal((batch, per))).astype(np.complex64)
+        host = torch.from_numpy(x.view(np.float32))
+        ins = rotation(torch, tuple(host.shape), torch.float32, host)
+        outs = [torch.empty_like(ins[0]) for _ in ins]
+        plan.exec_split(ins[0].data_ptr(), outs[0].data_ptr())
+        torch.cuda.synchronize()
+        got = outs[0][0].cpu().numpy().view(np.complex64).astype(np.complex128)
+        x0 = x[0].astype(np.complex128)
+        ref = np.fft.fft(x0) if dims == 1 else np.fft.fft2(x0.reshape(a, b)).reshape(-1)
+        err = rel(got, ref)
+        ms = graph_time(torch, lambda i, st: plan.exec_split(ins[i % len(ins)].data_ptr(),
+                                                             outs[i % len(ins)].data_ptr(), tcfft_amd.FORWARD, st),
+                        reps, 3)
+        npass = len(tcfft_amd.query_split(dims, a, b, batch))
+        res[name] = {"ms": round(ms, 5), "passes": npass, "rel_l2_vs_fp64": err,
+                     "hbm_frac_per_pass": round(16.0 * per * batch * npass / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS, 4)}
+        plan.destroy()
+        del ins, outs
+    return res
+
+
 def bench_uneven(torch, tcfft_amd, reps):
     """4096 fp16 signals, N = 2^U{8..16} (seed 0), drop-in order: one
     persistent autotuned launch vs one launch per size class (regular plans
@@ -753,6 +785,7 @@ def main():
         cfg["c2a_n2^20_b16"] = bench_plan1d(torch, tcfft_amd, 1 << 20, 16, reps, crng)
         cfg["c2b_2d_1024x1024_b8"] = bench_plan2d(torch, tcfft_amd, 1024, 1024, 8, reps, crng)
         cfg["c3_split_n2^16_b256"] = bench_split(torch, tcfft_amd, reps, ms_plain=ms)
+        cfg.update(bench_split_shapes(torch, tcfft_amd, reps))
         cfg["c4_uneven_4096_signals"] = bench_uneven(torch, tcfft_amd, reps)
         cfg["c5_fftconv_64x256x8192"] = bench_conv(torch, 5)
         cfg["r2c_n8192_16384_signals"] = bench_r2c(torch, tcfft_amd, reps)

@@ -356,7 +356,7 @@ __device__ __forceinline__ void split_pass_f(const float2* src, float2* dst, con
     constexpr int F = R / 16, G = 16 / F, LR = F == 1 ? 4 : F == 2 ? 5 : F == 4 ? 6 : 7;
     constexpr float sg = SIGN < 0 ? 1.f : -1.f;
     const int lane = lane_id(), c16 = lane & 15, g = lane >> 4;
-    const u32 wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const u32 wave = blockIdx.x * 4u + (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const u32 upq = a.q_log2 - 4u;  // log2 of units per signal
     // the stage-1 twiddle steps W_R^{4g+i} first (global loads complete in
     // issue order: a table value loaded after the data would wait for it)
@@ -370,14 +370,15 @@ __device__ __forceinline__ void split_pass_f(const float2* src, float2* dst, con
 #pragma unroll
     for (int cg = 0; cg < G; ++cg) {
         const u32 u = min(wave * G + cg, a.units - 1u);  // a ragged wave re-reads the last unit
-        const float2* p = src + ((size_t)(u >> upq) << (a.q_log2 + LR)) + ((u & ((1u << upq) - 1u)) << 4) + c16 +
-                          ((size_t)(4 * F * g) << a.q_log2);
+        // wave-uniform base + 32-bit lane offset (< 2^27 elements)
+        const float2* ub = src + ((size_t)(u >> upq) << (a.q_log2 + LR)) + ((u & ((1u << upq) - 1u)) << 4);
+        const u32 lo = (u32)c16 + ((u32)(4 * F * g) << a.q_log2);
 #pragma unroll
         for (int r0 = 0; r0 < F; ++r0)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const f32x2 v = __builtin_nontemporal_load(
-                    reinterpret_cast<const f32x2*>(p + ((size_t)(r0 + F * j) << a.q_log2)));
+                const float2* up = ub + ((size_t)(r0 + F * j) << a.q_log2);
+                const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(up + lo));
                 dr[cg * F + r0][j] = v[0];
                 di[cg * F + r0][j] = v[1];
             }
@@ -496,7 +497,7 @@ __device__ __forceinline__ void split_pass_256(const float2* src, float2* dst, c
                                                const float2* __restrict__ w4096) {
     constexpr float sg = SIGN < 0 ? 1.f : -1.f;
     const int lane = lane_id(), c16 = lane & 15, g = lane >> 4;
-    const u32 wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const u32 wave = blockIdx.x * 4u + (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const u32 upq = a.q_log2 - 4u;
     const u32 u = min(wave, a.units - 1u);
     const u32 q = ((u & ((1u << upq) - 1u)) << 4) + c16;
@@ -509,13 +510,15 @@ __device__ __forceinline__ void split_pass_256(const float2* src, float2* dst, c
     // rows r = r0 + 16 (4g + j): tile r0, word j
     float dr[16][4], di[16][4];
     {
-        const float2* p = src + sbase + q + ((size_t)(64 * g) << a.q_log2);
+        // wave-uniform base + 32-bit lane offset (< 2^27 elements)
+        const float2* ub = src + sbase + (q - c16);
+        const u32 lo = (u32)c16 + ((u32)(64 * g) << a.q_log2);
 #pragma unroll
         for (int r0 = 0; r0 < 16; ++r0)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const f32x2 v = __builtin_nontemporal_load(
-                    reinterpret_cast<const f32x2*>(p + ((size_t)(r0 + 16 * j) << a.q_log2)));
+                const float2* up = ub + ((size_t)(r0 + 16 * j) << a.q_log2);
+                const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(up + lo));
                 dr[r0][j] = v[0];
                 di[r0][j] = v[1];
             }
@@ -528,6 +531,8 @@ __device__ __forceinline__ void split_pass_256(const float2* src, float2* dst, c
         for (int j = 0; j < 4; ++j) m = fmaxf(m, fmaxf(fabsf(dr[t][j]), fabsf(di[t][j])));
     float s, inv;
     wave_scale(m, s, inv);
+    s = uniform_f(s);
+    inv = uniform_f(inv);
     {
         const Dft16Split F = make_dft16_split(lane, SIGN);
         cf step[4], rec[4];

@@ -167,6 +167,26 @@ def test_graph_split_precision(chunk):
     plan.destroy()
 
 
+@pytest.mark.parametrize("shape", [(1, 1 << 20, 0, 3), (2, 512, 1024, 2)], ids=["1d-2^20x3", "2d-512x1024x2"])
+def test_graph_split_precision_prepared(shape):
+    """Prepared split plans (three or four column passes through the plan's
+    workspace), captured in place: every replay restores the input first."""
+    dims, a, b, batch = shape
+    note(f"[gpu] graph: prepared split {shape}")
+    rng = np.random.default_rng(8)
+    plan = tcfft_amd.Plan1D(a, batch) if dims == 1 else tcfft_amd.Plan2D(a, b, batch)
+    plan.prepare_split()
+    total = a * (b or 1) * batch
+    x = torch.from_numpy(rng.standard_normal((total, 2)).astype(np.float32)).cuda()
+    want = x.clone()
+    plan.exec_split(want.data_ptr(), want.data_ptr())
+    torch.cuda.synchronize()
+    buf = torch.empty_like(x)
+    replay_check(lambda st: plan.exec_split(buf.data_ptr(), buf.data_ptr(), tcfft_amd.FORWARD, st),
+                 lambda: buf.copy_(x), buf, want, lambda: plan.exec_split(x.data_ptr(), torch.empty_like(x).data_ptr()))
+    plan.destroy()
+
+
 def test_graph_fused_convolution():
     note("[gpu] graph: fused convolution n=8192, 4 x 8 channels")
     rng = np.random.default_rng(9)
