@@ -101,10 +101,16 @@ def graph_time(torch, launch, reps, replays=1):
     return float(np.median(times))
 
 
-def eager_time(torch, fn, reps):
+def eager_time(torch, fn, reps, spin=SPIN_CYCLES):
+    """ms per fn(i) over reps eager calls on the current stream.  As in
+    graph_time, a spin of `spin` GPU cycles is queued ahead of the start
+    event, so that the region holds the calls' GPU work back to back and not
+    the host's latency before the first launch (callers whose Python-side
+    call is slow pass a longer spin)."""
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
+    torch.cuda._sleep(spin)
     a.record()
     for i in range(reps):
         fn(i)
@@ -507,8 +513,10 @@ def bench_conv(torch, reps):
 
     fwd(0)
     fwd_bwd(0)
-    ms_f = eager_time(torch, fwd, reps)
-    ms_fb = eager_time(torch, fwd_bwd, max(2, reps // 2))
+    # a ~2 ms spin: the layer's Python forward and backward take far longer
+    # on the host than the default spin covers
+    ms_f = eager_time(torch, fwd, reps, spin=5_000_000)
+    ms_fb = eager_time(torch, fwd_bwd, max(2, reps // 2), spin=5_000_000)
     sig = B * C
     fft_flops = 5.0 * n * math.log2(n) * sig
     return {"ms_forward": round(ms_f, 4), "ms_forward_backward": round(ms_fb, 4),
@@ -787,7 +795,7 @@ def main():
         cfg["c3_split_n2^16_b256"] = bench_split(torch, tcfft_amd, reps, ms_plain=ms)
         cfg.update(bench_split_shapes(torch, tcfft_amd, reps))
         cfg["c4_uneven_4096_signals"] = bench_uneven(torch, tcfft_amd, reps)
-        cfg["c5_fftconv_64x256x8192"] = bench_conv(torch, 5)
+        cfg["c5_fftconv_64x256x8192"] = bench_conv(torch, 10)
         cfg["r2c_n8192_16384_signals"] = bench_r2c(torch, tcfft_amd, reps)
         cfg["hipfft_fp16_n2^16_b256"] = hipfft_fp16(torch, [n], batch, reps)
         cfg["hipfft_fp16_n4096_b1024"] = hipfft_fp16(torch, [4096], 1024, reps)

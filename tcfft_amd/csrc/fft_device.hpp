@@ -253,6 +253,27 @@ __device__ __forceinline__ void store4(u32* __restrict__ p, const u32* w) {
     *reinterpret_cast<u32x4*>(p) = v;
 }
 
+// Access at a wave-uniform base plus a 32-bit per-lane byte offset: the
+// compiler then uses the SGPR-base form of global loads and stores (base in
+// SGPRs, offset in one VGPR) instead of a 64-bit address per lane, which
+// for a strided pass costs three VALU ops and two VGPRs per access.
+template <class T>
+__device__ __forceinline__ T ld_at(const void* base, u32 off_bytes) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off_bytes);
+}
+template <class T>
+__device__ __forceinline__ T ld_at_nt(const void* base, u32 off_bytes) {
+    return __builtin_nontemporal_load(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + off_bytes));
+}
+template <class T>
+__device__ __forceinline__ void st_at(void* base, u32 off_bytes, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off_bytes) = v;
+}
+template <class T>
+__device__ __forceinline__ void st_at_nt(void* base, u32 off_bytes, T v) {
+    __builtin_nontemporal_store(v, reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off_bytes));
+}
+
 // Small radix-T DFT in FP32 registers (T = 2, 4, 8), in place on v[0..T).
 template <int T>
 __device__ __forceinline__ void small_dft(cf* v, int sign);

@@ -690,7 +690,12 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
     const u32 cg_log2 = s_log2 - 4u;  // column groups of 16 per block: s / 16
     const u32 blk = grp >> cg_log2;
     const u32 col0 = (grp & ((1u << cg_log2) - 1u)) * 16u;
-    const size_t blk_base = (((size_t)blk * (256u * W)) << s_log2) + col0 + (u32)c16;
+    // wave-uniform bases and 32-bit lane byte offsets (at most 192 rows of
+    // s <= 2^19 elements into the block), so that no load or store needs a
+    // 64-bit per-lane address
+    u32* const blk_u = buf + (((size_t)blk * (256u * W)) << s_log2) + col0;
+    const u32 lane_ld = ((u32)c16 + ((4u * g) << s_log2)) * 4u;
+    const u32 lane_st = ((u32)c16 + ((64u * g) << s_log2)) * 4u;
 
     // table values before the data (global loads complete in issue order)
     float2 t_step[4], t_rec0[4];
@@ -706,7 +711,10 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
 #pragma unroll
     for (int t = 0; t < 16; ++t)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) d[t][i] = buf[blk_base + ((size_t)(16u * t + 4u * g + i + 256u * w) << s_log2)];
+        for (int i = 0; i < 4; ++i) {
+            const u32* up = blk_u + ((size_t)(16u * t + i + 256u * w) << s_log2);
+            d[t][i] = ld_at<u32>(up, lane_ld);
+        }
     __builtin_amdgcn_sched_barrier(0);
 
     const Dft16Op op = make_dft16(lane, SIGN);
@@ -757,9 +765,12 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
 #pragma unroll
                 for (int src = 0; src < W; ++src) x[src] = unpack(v[src][i]);
                 small_dft<W>(x, SIGN);
-                const u32 row = k0 + 16u * (4u * g + i);
+                // row k0 + 16 (4g + i) + 256 k2: the 64 g part is the lane's
 #pragma unroll
-                for (int k2 = 0; k2 < W; ++k2) buf[blk_base + ((size_t)(row + 256u * k2) << s_log2)] = pack(x[k2]);
+                for (int k2 = 0; k2 < W; ++k2) {
+                    u32* up = blk_u + ((size_t)(k0 + 16u * i + 256u * k2) << s_log2);
+                    st_at<u32>(up, lane_st, pack(x[k2]));
+                }
             }
         }
     }
