@@ -19,7 +19,15 @@ Real input (n = 2^9..2^27):
     c2r(X, n)        y = sum over the Hermitian extension of X, n * irfft
     irfft(X, n)      c2r(X, n) / n (the scale is applied in fp32)
 
-Their backward passes are each other's kernels: viewed as real-linear maps,
+Near-FP32 accuracy (split precision, complex64 [..., n] or float32 [..., n, 2],
+n = 2^8..2^27):
+
+    fft32(x)         X = F x
+    fft32_adjoint(x) Y = F^H x
+
+with the same adjoint pairing as fft / fft_adjoint in their backward passes.
+
+The real-input transforms' backward passes are each other's kernels: viewed as real-linear maps,
 the transpose of rfft is c2r applied to the gradient with bins 1..n/2-1
 halved (c2r counts those bins twice), and the transpose of c2r is rfft of the
 gradient with bins 1..n/2-1 doubled.
@@ -124,6 +132,68 @@ def ifft(x: torch.Tensor) -> torch.Tensor:
     """Normalised inverse, F^H x / n (the scale is applied in fp32)."""
     n = x.shape[-2]
     return (fft_adjoint(x).float() / n).half()
+
+
+_SPLANS: OrderedDict = OrderedDict()
+
+
+def _splan(n: int, batch: int, device: torch.device) -> Plan1D:
+    def make():
+        p = Plan1D(n, batch)
+        p.device_index = device.index
+        p.prepare_split()  # tables and workspace here, not in the exec
+        return p
+    return _cached(_SPLANS, (n, batch, device.index), make)
+
+
+def _exec_split(x: torch.Tensor, direction: int) -> torch.Tensor:
+    """Split-precision transform along the n axis of an FP32 [..., n, 2] or
+    complex64 [..., n] CUDA tensor; returns the same dtype and shape."""
+    if x.device.type != "cuda":
+        raise RuntimeError("tcfft_amd.torch_fft needs a CUDA (HIP) tensor")
+    cplx = x.dtype == torch.complex64
+    v = torch.view_as_real(x) if cplx else x
+    if v.dtype != torch.float32 or v.dim() < 2 or v.shape[-1] != 2:
+        raise RuntimeError("expected a complex64 [..., n] or float32 [..., n, 2] tensor")
+    v = v.contiguous()
+    n = v.shape[-2]
+    batch = v.numel() // (2 * n)
+    out = torch.empty_like(v)
+    _splan(n, batch, v.device).exec_split(v.data_ptr(), out.data_ptr(), direction,
+                                          torch.cuda.current_stream(v.device).cuda_stream)
+    return torch.view_as_complex(out) if cplx else out
+
+
+class _FFT32(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        return _exec_split(x, FORWARD)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _exec_split(g.contiguous(), INVERSE)
+
+
+class _FFT32Adjoint(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x):
+        return _exec_split(x, INVERSE)
+
+    @staticmethod
+    def backward(ctx, g):
+        return _exec_split(g.contiguous(), FORWARD)
+
+
+def fft32(x: torch.Tensor) -> torch.Tensor:
+    """Differentiable forward FFT at near-FP32 accuracy (split precision, FP16
+    hi+lo on MFMA) along the last complex axis: complex64 [..., n] or float32
+    [..., n, 2], n = 2^8..2^27."""
+    return _FFT32.apply(x)
+
+
+def fft32_adjoint(x: torch.Tensor) -> torch.Tensor:
+    """Differentiable unnormalised inverse (adjoint) of fft32, F^H x."""
+    return _FFT32Adjoint.apply(x)
 
 
 def _rplan(n: int, batch: int, device: torch.device) -> PlanR2C:

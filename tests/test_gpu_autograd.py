@@ -112,6 +112,42 @@ def test_fft_backward_matches_torch_fft(n):
     assert rel(to_c(x2.grad), xt2.grad.numpy()) <= TOL
 
 
+@pytest.mark.parametrize("n", [256, 4096, 1 << 17])
+def test_fft32_values_and_gradients_vs_torch_fft(n):
+    """Split-precision fft32 / fft32_adjoint on complex64 and float32 [..., n, 2]
+    tensors: values and gradients against complex128 torch.fft autograd
+    within 1e-6 (near-FP32: the bound is three orders below the fp16 one)."""
+    note(f"[gpu] fft32 n={n}")
+    g = torch.Generator().manual_seed(n)
+    xc = torch.complex(torch.randn(3, n, generator=g), torch.randn(3, n, generator=g)).cuda().requires_grad_(True)
+    w = torch.complex(torch.randn(3, n, generator=g), torch.randn(3, n, generator=g)).cuda()
+    X = torch_fft.fft32(xc)
+    assert X.dtype == torch.complex64 and X.shape == xc.shape
+    (X * w.conj()).real.sum().backward()
+    xt = xc.detach().cpu().to(torch.complex128).requires_grad_(True)
+    Xt = torch.fft.fft(xt, dim=-1)
+    (Xt * w.cpu().to(torch.complex128).conj()).real.sum().backward()
+    assert rel(X.detach().cpu().numpy(), Xt.detach().numpy()) <= 1e-6
+    assert rel(xc.grad.cpu().numpy(), xt.grad.numpy()) <= 1e-6
+    # float32 [..., n, 2] layout and the adjoint, whose backward is fft32
+    xr = torch.view_as_real(xc.detach().clone()).requires_grad_(True)
+    Y = torch_fft.fft32_adjoint(xr)
+    assert Y.dtype == torch.float32 and Y.shape == xr.shape
+    wr = torch.view_as_real(w)
+    (Y * wr).sum().backward()
+    xt2 = xc.detach().cpu().to(torch.complex128).requires_grad_(True)
+    Yt = torch.fft.ifft(xt2, dim=-1) * n
+    wc = w.cpu().to(torch.complex128)
+    (Yt.real * wc.real + Yt.imag * wc.imag).sum().backward()
+    assert rel(to_c(Y.detach()), Yt.detach().numpy()) <= 1e-6
+    assert rel(to_c(xr.grad), xt2.grad.numpy()) <= 1e-6
+
+
+def test_fft32_rejects_fp16():
+    with pytest.raises(RuntimeError):
+        torch_fft.fft32(torch.zeros(4, 256, 2, dtype=torch.float16, device="cuda"))
+
+
 def reference_conv(x_c, h_c, n):
     """FP64 y = F^H (H * F x) / n."""
     return np.fft.ifft(np.fft.fft(x_c, axis=-1) * h_c, axis=-1)
