@@ -227,7 +227,8 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * for a strided pass of radix 512..2048.
  * TCFFT_OPT_SPLIT_DELAY (0..64, default 2): the same start delay (3 groups)
  * for the two passes of tcfft_exec_split.
- * TCFFT_OPT_ROWS_WAVES (1, 2 or 4, default 1): waves per workgroup of a
+ * TCFFT_OPT_ROWS_WAVES (1, 2 or 4; 0, the default, = 4 for 4096-point lines
+ * over at least 32 waves per compute unit, else 1): waves per workgroup of a
  * contiguous-row first pass (lines of 256..4096 points, one wave per 16
  * tiles of 256 points), i.e. how the waves are grouped onto compute units.
  * The schedule never changes results. */

@@ -213,10 +213,11 @@ struct tcfft_plan_s {
     int cus = 0;          // compute units of the plan's device
     int big_delay = -1;   // start schedule of the first-pass kernels: -1 = default
     int big_groups = 3;
-    // waves per workgroup of the contiguous-row passes (1, 2, 4); one-wave
-    // workgroups measured fastest (4096 x 1024: 7.2 us against 8.1 with 4,
-    // 1024-point rows of 2D 1024^2 x 8: 14.7 against 15.1)
-    int rows_waves = 1;
+    // waves per workgroup of the contiguous-row passes (1, 2, 4; 0 = by the
+    // pass, see rows_wg); one-wave workgroups measured fastest for one round
+    // of waves (4096 x 1024: 7.2 us against 8.1 with 4, 1024-point rows of
+    // 2D 1024^2 x 8: 14.7 against 15.1)
+    int rows_waves = 0;
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
     int split_delay = 2;  // start delay of the split-precision passes (3 groups; 2^16 x 256: 110.6 -> 106.6 us)
     // Split precision at every size (tcfft_plan_prepare_split): the column
@@ -417,6 +418,19 @@ tcfft::StartSched big_sched(const tcfft_plan_s* p, int W) {
 // only when the grid is at most one 4-wave workgroup per CU (4096 x 1024:
 // 10.0 -> 9.2 us); with more workgroups per CU they desynchronise by
 // themselves and a delay only costs time (1024 x 8192: 15.9 -> 16.2 us).
+// Waves per workgroup of a contiguous-row first pass: the plan's setting, or
+// by default 4 for 4096-point lines over at least 32 waves per CU and 1
+// otherwise.  Measured (rotated buffers): 4096 x 1024 7.7 us with one wave
+// against 8.6 with four; even at 2048-4096 lines; four ahead from 8192
+// lines (47.7 against 48.6 us) to 65536 (352 against 369 us); for 2048-point
+// and shorter lines one wave stays ahead at every batch (2048 x 131072: 369
+// against 403 us).
+uint32_t rows_wg(const tcfft_plan_s* p, const Pass& ps) {
+    if (p->rows_waves > 0) return (uint32_t)p->rows_waves;
+    const uint32_t waves = (ps.n_tiles + 15u) / 16u;
+    return ps.radix == 16 && waves >= 32u * (uint32_t)p->cus ? 4u : 1u;
+}
+
 tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps) {
     const uint32_t waves = (ps.n_tiles + 15u) / 16u;  // at most one wave per SIMD: a delay of 1
     const int delay = p->big_delay >= 0 ? p->big_delay : (waves <= 4u * (uint32_t)p->cus ? 1 : 0);
@@ -431,7 +445,7 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
         hipError_t e;
         if (ps.kind == 0)
             e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, rows_sched(p, ps),
-                                   (uint32_t)p->rows_waves, st);
+                                   rows_wg(p, ps), st);
         else if (ps.kind == 2)
             e = tcfft::launch_big(ps.radix, buf, ps.n_tiles, sign, p->w4096, ps.big_hi, ps.big_lo, ps.big_M_log2 / 2,
                                   epi, big_sched(p, ps.radix), st);
@@ -1005,7 +1019,7 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
         plan->split_delay = value;
         return TCFFT_SUCCESS;
     case TCFFT_OPT_ROWS_WAVES:
-        if (value != 1 && value != 2 && value != 4) return TCFFT_INVALID_VALUE;
+        if (value != 0 && value != 1 && value != 2 && value != 4) return TCFFT_INVALID_VALUE;
         plan->rows_waves = value;
         return TCFFT_SUCCESS;
     default: return TCFFT_INVALID_VALUE;

@@ -111,11 +111,12 @@ def test_column_pass_delay_bit_identical(shape, batch):
     plan.destroy()
 
 
-@pytest.mark.parametrize("n,batch", [(4096, 1024), (4096, 37), (1024, 300), (256, 4099), (1 << 20, 3)])
+@pytest.mark.parametrize("n,batch", [(4096, 1024), (4096, 37), (4096, 8192), (1024, 300), (256, 4099), (1 << 20, 3)])
 def test_rows_workgroup_size_bit_identical(n, batch):
     """Waves per workgroup of a contiguous-row first pass (TCFFT_OPT_ROWS_WAVES
-    1, 2, 4; 2^20 with a 4096-point first pass) never change results; other
-    values are rejected."""
+    0 = by the pass, 1, 2, 4; 4096 x 8192 is a batch whose default is 4; 2^20
+    with a 4096-point first pass) never change results; other values are
+    rejected."""
     note(f"[gpu] rows workgroup size n={n} batch={batch}")
     x, d = dropin_input(n, batch, n + 3 * batch)
     plan = tcfft_amd.Plan1D(n, batch, first_log2=12) if n > 65536 else tcfft_amd.Plan1D(n, batch)
@@ -127,7 +128,7 @@ def test_rows_workgroup_size_bit_identical(n, batch):
     plan.exec(ref.data_ptr())
     torch.cuda.synchronize()
     assert rel_l2(ref, np.fft.fft(x, axis=1)) <= (1e-3 if n <= 65536 else 1.5e-3)
-    for wg in (1, 2, 4):
+    for wg in (1, 2, 4, 0):
         plan.set_option(tcfft_amd.OPT_ROWS_WAVES, wg)
         for delay in (0, 1):
             plan.set_option(tcfft_amd.OPT_BIG_DELAY, delay)
@@ -135,7 +136,7 @@ def test_rows_workgroup_size_bit_identical(n, batch):
             plan.exec(t.data_ptr())
             torch.cuda.synchronize()
             assert torch.equal(t, ref), (wg, delay)
-    for bad in (0, 3, 8):
+    for bad in (-1, 3, 8):
         with pytest.raises(tcfft_amd.TcfftError):
             plan.set_option(tcfft_amd.OPT_ROWS_WAVES, bad)
     plan.destroy()
