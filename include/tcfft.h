@@ -145,9 +145,10 @@ int tcfft_exec_split(tcfft_plan plan, const void *in, void *out, int direction, 
  * of 16..256 rows, 2..4 per axis), uploads their twiddle tables and
  * allocates a workspace of 8 bytes per element, so that tcfft_exec_split
  * allocates nothing (graph-capturable) and runs in place.  One workspace per
- * plan: split calls on one plan that use it (three or more passes, or in
- * place) must not overlap, and this call must not run concurrently with any
- * other call on the plan.  Idempotent.  No counterpart in the reference. */
+ * plan: split calls on one plan must not overlap (out-of-place calls on a 1D
+ * plan of at most 65536 points do not use it and may), and this call must not
+ * run concurrently with any other call on the plan.  Idempotent.  No
+ * counterpart in the reference. */
 int tcfft_plan_prepare_split(tcfft_plan plan);
 
 /* The split-precision pass list tcfft_plan_prepare_split builds for a shape
