@@ -133,6 +133,29 @@ def test_split_largest_round_trip_and_bins(k):
     plan.destroy()
 
 
+@pytest.mark.parametrize("n", [256, 4096, 1 << 17])
+def test_split_tiny_and_mixed_scales(n):
+    """Signals below 2^-114 (the clamped unit scale, whose inverse must still
+    undo it exactly) beside signals of magnitude 1 and 1e30: every unit of 16
+    columns takes its own scale, so each signal keeps the same bound; at
+    n = 256 a wave's 16 units are 16 different signals."""
+    note(f"[gpu] split tiny/mixed scales n={n}")
+    rng = np.random.default_rng(n + 7)
+    scales = np.array([1e-36, 3e-37, 1.0, 1e30, 5e-36, 1e-3] * 3)[:, None]
+    b = scales.shape[0]
+    x = ((rng.standard_normal((b, n)) + 1j * rng.standard_normal((b, n))) * scales).astype(np.complex64)
+    plan = tcfft_amd.Plan1D(n, b)
+    plan.prepare_split()
+    for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
+        got = run(plan, x, direction, False)
+        x64 = x.astype(np.complex128)
+        ref = np.fft.fft(x64, axis=-1) if direction < 0 else np.fft.ifft(x64, axis=-1) * n
+        errs = per_signal_err(got, ref)
+        note(f"  dir={direction} per-signal rel_l2 max={errs.max():.3e}")
+        assert errs.max() <= TOL
+    plan.destroy()
+
+
 def test_split_prepared_65536_in_place_and_identical():
     """The 65536-point kernel pair on a prepared plan: in place through the
     workspace, bit-identical to the out-of-place call."""
