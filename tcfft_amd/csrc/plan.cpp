@@ -414,6 +414,16 @@ tcfft::StartSched big_sched(const tcfft_plan_s* p, int W) {
     return tcfft::StartSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
 }
 
+// The same for the natural-order single-pass kernel.  Its workgroups stage
+// the whole signal through LDS before any compute, so the 65536-point kernel
+// wants a longer stagger: 2^16 x 256 natural (3 groups) with delays 2, 4, 6
+// took 44.8, 42.8, 41.3 us on one box and 43.7, 42.5, 43.4 us on another;
+// for W = 2, 4, 8 the drop-in defaults stay within 2% of the best.
+tcfft::StartSched nat_big_sched(const tcfft_plan_s* p, int W) {
+    if (p->big_delay >= 0 || W != 16) return big_sched(p, W);
+    return tcfft::StartSched{4u, (uint32_t)p->big_groups, 0u};
+}
+
 // Start schedule of the contiguous-row first pass.  By default a delay of 1
 // only when the grid is at most one 4-wave workgroup per CU (4096 x 1024:
 // 10.0 -> 9.2 us); with more workgroups per CU they desynchronise by
@@ -1213,7 +1223,7 @@ int exec_natural_staged(tcfft_plan plan, const void* in, void* out, int directio
                                   : tcfft::launch_big_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
                                                           plan->w4096, first.big_hi, first.big_lo,
                                                           first.big_M_log2 / 2, nx_log2, x_pos,
-                                                          big_sched(plan, first.radix), st);
+                                                          nat_big_sched(plan, first.radix), st);
         if (e != hipSuccess) return hip_fail(e);
         return run_passes(plan, (uint32_t*)out, direction, st, 1);
     }
