@@ -233,6 +233,11 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * over at least 32 waves per compute unit, else 1): waves per workgroup of a
  * contiguous-row first pass (lines of 256..4096 points, one wave per 16
  * tiles of 256 points), i.e. how the waves are grouped onto compute units.
+ * TCFFT_OPT_FIRST_CACHED (-1, the default: cached while the transform's
+ * buffer is at most 256 MiB; 0 = never; 1 = always): whether the first pass
+ * of a multi-pass plan stores its output through the caches, so that the
+ * next pass finds much of it in the 256 MiB last-level cache (MALL), instead
+ * of with non-temporal stores.
  * The schedule never changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,
@@ -240,7 +245,8 @@ enum tcfft_option {
     TCFFT_OPT_BIG_GROUPS = 3,
     TCFFT_OPT_COLS_DELAY = 4,
     TCFFT_OPT_SPLIT_DELAY = 5,
-    TCFFT_OPT_ROWS_WAVES = 6
+    TCFFT_OPT_ROWS_WAVES = 6,
+    TCFFT_OPT_FIRST_CACHED = 7
 };
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 

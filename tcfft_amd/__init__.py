@@ -54,6 +54,7 @@ OPT_BIG_GROUPS = 3
 OPT_COLS_DELAY = 4
 OPT_SPLIT_DELAY = 5
 OPT_ROWS_WAVES = 6
+OPT_FIRST_CACHED = 7
 
 
 class TcfftError(RuntimeError):

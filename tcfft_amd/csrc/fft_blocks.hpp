@@ -240,9 +240,12 @@ __device__ __forceinline__ void big_phase1(u32 (&d)[16][4], u32 (&e)[16][4], con
 // Phase 2: LDS exchange of the W lines (layout [w][sigma(c)]) and the
 // radix-W stage over w, storing X[c + 4096*k3] at sbase (4 adjacent columns
 // per lane).  With SCALE the FP32 results are multiplied by `scale` first.
+// The stores are non-temporal, or with CACHED cached, for a pass whose
+// output the next pass reads back from the MALL (a template parameter: the
+// compiler merges a runtime choice between the two into one cached store).
 // Every barrier is a workgroup barrier: all groups of a workgroup run the
 // same sequence.
-template <int W, int ROUNDS, bool EPI, int SIGN, bool SCALE = false>
+template <int W, int ROUNDS, bool EPI, int SIGN, bool SCALE = false, bool CACHED = false>
 __device__ __forceinline__ void big_phase2(u32 (&e)[16][4], int w, u32* lds, u32* __restrict__ sbase,
                                            const BigTw& bt, const Dft16Op& op, bool active, float scale = 1.f) {
     static_assert(!SCALE || W < 16, "output scaling is implemented for the VALU radix-W stage");
@@ -293,7 +296,12 @@ __device__ __forceinline__ void big_phase2(u32 (&e)[16][4], int w, u32* lds, u32
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     u32 q[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
-                    if (active) store4_nt(sbase + c0 + 4096u * (4u * g + i), q);
+                    if (active) {
+                        if constexpr (CACHED)
+                            store4(sbase + c0 + 4096u * (4u * g + i), q);
+                        else
+                            store4_nt(sbase + c0 + 4096u * (4u * g + i), q);
+                    }
                 }
             }
         } else {
@@ -326,7 +334,12 @@ __device__ __forceinline__ void big_phase2(u32 (&e)[16][4], int w, u32* lds, u32
                 const u32 c0 = 2048u * r + 4u * lane + 256u * (q + QN * w);
 #pragma unroll
                 for (int s = 0; s < W; ++s)
-                    if (active) store4_nt(sbase + c0 + 4096u * s, o[s]);
+                    if (active) {
+                        if constexpr (CACHED)
+                            store4(sbase + c0 + 4096u * s, o[s]);
+                        else
+                            store4_nt(sbase + c0 + 4096u * s, o[s]);
+                    }
             }
         }
     }
@@ -376,7 +389,7 @@ __device__ __forceinline__ void big_phase2_regs(u32 (&e)[16][4], int w, u32* lds
 // One signal of N = 4096*W at sbase, processed by the W waves of a group
 // (this wave is number w) with lds = the group's W*COLS words of LDS.  sig
 // is the signal's index (epilogue only).
-template <int W, int ROUNDS, bool EPI, int SIGN>
+template <int W, int ROUNDS, bool EPI, int SIGN, bool CACHED = false>
 __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* lds, u32 sig,
                                          const float2* __restrict__ w4096, const float2* __restrict__ wn_hi,
                                          const float2* __restrict__ wn_lo, u32 wn_lo_bits, const EpiParams& epi,
@@ -399,7 +412,7 @@ __device__ __forceinline__ void big_body(u32* __restrict__ sbase, int w, u32* ld
     const Dft16Op op = make_dft16(lane, SIGN);
     u32 e[16][4];
     big_phase1<SIGN>(d, e, bt, op);
-    big_phase2<W, ROUNDS, EPI, SIGN>(e, w, lds, sbase, bt, op, active);
+    big_phase2<W, ROUNDS, EPI, SIGN, false, CACHED>(e, w, lds, sbase, bt, op, active);
 }
 
 }  // namespace tcfft

@@ -30,6 +30,12 @@ struct StartSched {
     uint32_t delay;
     uint32_t groups;
     uint32_t resident;
+    // Not a schedule but a launch policy carried with it: 1 = the pass's
+    // output stores go through the caches (k_big / k_big_nat, and k_rows /
+    // k_rows_nat for 4096-point lines, whose stores are otherwise
+    // non-temporal; a separate kernel instance).  Set for a first pass that
+    // a further pass reads back while the buffer fits in the MALL.
+    uint32_t cached_out;
 };
 
 // First pass: contiguous lines of 256*T elements (T = 1, 2, 4, 8, 16), one

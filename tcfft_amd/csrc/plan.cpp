@@ -220,6 +220,9 @@ struct tcfft_plan_s {
     int rows_waves = 0;
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
     int split_delay = 2;  // start delay of the split-precision passes (3 groups; 2^16 x 256: 110.6 -> 106.6 us)
+    // first pass's output stores of a multi-pass plan: -1 = cached while the
+    // buffer is at most 256 MiB (cached_out), 0 = non-temporal, 1 = cached
+    int first_cached = -1;
     // Split precision at every size (tcfft_plan_prepare_split): the column
     // passes in order, and a workspace of total * 8 bytes (FP32 complex)
     // for the passes whose transposed store cannot go in place.
@@ -447,18 +450,34 @@ tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps) {
     return tcfft::StartSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
 }
 
+// Whether pass i stores its output through the caches: only the first pass of
+// a multi-pass plan (the next pass reads its output back), and by default
+// only while the buffer is at most the 256 MiB of the MALL.  Measured per
+// exec, non-temporal -> cached: 2^20 x 16 (64 MiB) 54.4 -> 46.3 us, 2^21 x 16
+// 101 -> 83.5, 2^22 x 16 (256 MiB) 201 -> 167, 2D 4096 x 4096 84.7 -> 73.5;
+// at 2^24 x 8 (512 MiB) cached stores cost 5% (481 -> 504 us).  A single-pass
+// plan keeps its non-temporal stores (2^16 x 256: 27.7 us, 29.5 cached).
+uint32_t cached_out(const tcfft_plan_s* p, size_t i) {
+    if (i != 0 || p->lay.passes.size() < 2 || p->first_cached == 0) return 0u;
+    return p->first_cached == 1 || p->total * 4u <= ((size_t)256 << 20) ? 1u : 0u;
+}
+
 int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t first = 0, size_t count = ~(size_t)0) {
     const size_t end = count > p->lay.passes.size() - first ? p->lay.passes.size() : first + count;
     for (size_t i = first; i < end; ++i) {
         const Pass& ps = p->lay.passes[i];
         const EpiParams* epi = ps.has_epi ? &ps.epi : nullptr;
         hipError_t e;
-        if (ps.kind == 0)
-            e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, rows_sched(p, ps),
-                                   rows_wg(p, ps), st);
-        else if (ps.kind == 2)
+        if (ps.kind == 0) {
+            tcfft::StartSched sc = rows_sched(p, ps);
+            sc.cached_out = cached_out(p, i);
+            e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, sc, rows_wg(p, ps), st);
+        } else if (ps.kind == 2) {
+            tcfft::StartSched sc = big_sched(p, ps.radix);
+            sc.cached_out = cached_out(p, i);
             e = tcfft::launch_big(ps.radix, buf, ps.n_tiles, sign, p->w4096, ps.big_hi, ps.big_lo, ps.big_M_log2 / 2,
-                                  epi, big_sched(p, ps.radix), st);
+                                  epi, sc, st);
+        }
         else if (ps.kind == 3)
             e = epi ? hipErrorInvalidValue
                     : tcfft::launch_cols_big(ps.radix / 256, buf, ps.s_log2, ps.outer, sign, p->w4096,
@@ -1032,6 +1051,10 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
         if (value != 0 && value != 1 && value != 2 && value != 4) return TCFFT_INVALID_VALUE;
         plan->rows_waves = value;
         return TCFFT_SUCCESS;
+    case TCFFT_OPT_FIRST_CACHED:
+        if (value < -1 || value > 1) return TCFFT_INVALID_VALUE;
+        plan->first_cached = value;
+        return TCFFT_SUCCESS;
     default: return TCFFT_INVALID_VALUE;
     }
 }
@@ -1218,12 +1241,13 @@ int exec_natural_staged(tcfft_plan plan, const void* in, void* out, int directio
             std::vector<int> rx(L0.rad_x.rbegin(), L0.rad_x.rend());
             x_pos = rev_spec(rx);
         }
+        tcfft::StartSched sc = fused_rows ? rows_sched(plan, first) : nat_big_sched(plan, first.radix);
+        sc.cached_out = cached_out(plan, 0);
         hipError_t e = fused_rows ? tcfft::launch_rows_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
-                                                           plan->w4096, nx_log2, x_pos, rows_sched(plan, first), st)
+                                                           plan->w4096, nx_log2, x_pos, sc, st)
                                   : tcfft::launch_big_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
                                                           plan->w4096, first.big_hi, first.big_lo,
-                                                          first.big_M_log2 / 2, nx_log2, x_pos,
-                                                          nat_big_sched(plan, first.radix), st);
+                                                          first.big_M_log2 / 2, nx_log2, x_pos, sc, st);
         if (e != hipSuccess) return hip_fail(e);
         return run_passes(plan, (uint32_t*)out, direction, st, 1);
     }

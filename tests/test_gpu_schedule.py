@@ -203,3 +203,42 @@ def test_natural_rows_schedule_bit_identical(n, batch):
         torch.cuda.synchronize()
         assert torch.equal(out, ref), d
     plan.destroy()
+
+
+@pytest.mark.parametrize("shape", [("1d", 1 << 20, 4, 16), ("1d", 1 << 18, 8, 12), ("2d", 1024, 512, 2),
+                                   ("2d", 4096, 256, 1), ("1d", 65536, 4, 16)])
+def test_first_pass_cached_bit_identical(shape):
+    """Cached or non-temporal first-pass stores (TCFFT_OPT_FIRST_CACHED -1 =
+    by buffer size, 0, 1) give bit-identical output, on digit-reversed and on
+    natural-order input; 1D natural output matches FP64 numpy; values outside
+    -1..1 are rejected."""
+    kind, a, b, c = shape
+    note(f"[gpu] first pass cached {shape}")
+    rng = np.random.default_rng(a + b)
+    if kind == "1d":
+        plan = tcfft_amd.Plan1D(a, b, first_log2=c) if a > 65536 else tcfft_amd.Plan1D(a, b)
+        total = a * b
+    else:
+        plan = tcfft_amd.Plan2D(a, b, c)
+        total = a * b * c
+    x = torch.from_numpy(rng.uniform(-1, 1, (total, 2)).astype(np.float16)).cuda()
+    outs = {}
+    for v in (-1, 0, 1):
+        plan.set_option(tcfft_amd.OPT_FIRST_CACHED, v)
+        t = x.clone()
+        plan.exec(t.data_ptr())
+        o = torch.empty_like(x)
+        plan.exec_natural(x.data_ptr(), o.data_ptr())
+        torch.cuda.synchronize()
+        outs[v] = (t, o)
+    for v in (0, 1):
+        assert torch.equal(outs[v][0], outs[-1][0]), v
+        assert torch.equal(outs[v][1], outs[-1][1]), v
+    if kind == "1d":
+        h = x.cpu().numpy().astype(np.float64).reshape(b, a, 2)
+        want = np.fft.fft(h[..., 0] + 1j * h[..., 1], axis=1)
+        assert rel_l2(outs[-1][1].view(b, a, 2), want) <= (1e-3 if a <= 65536 else 1.5e-3)
+    for bad in (-2, 2):
+        with pytest.raises(tcfft_amd.TcfftError):
+            plan.set_option(tcfft_amd.OPT_FIRST_CACHED, bad)
+    plan.destroy()
