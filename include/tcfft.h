@@ -238,6 +238,12 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * of a multi-pass plan stores its output through the caches, so that the
  * next pass finds much of it in the 256 MiB last-level cache (MALL), instead
  * of with non-temporal stores.
+ * TCFFT_OPT_MALL_CHUNK (MiB, 16..65536; 0 = off; default 256): a multi-pass
+ * transform whose buffer exceeds this runs all its passes on one chunk of
+ * whole signals (images) of at most this size before the next, so that the
+ * later passes read the earlier ones' output from the MALL.  Plans with a
+ * radix-512..2048 column pass (2D nx = 512..2048, 1D 2^25..2^27) and batches
+ * whose single signal exceeds the size are not chunked.
  * The schedule never changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,
@@ -246,7 +252,8 @@ enum tcfft_option {
     TCFFT_OPT_COLS_DELAY = 4,
     TCFFT_OPT_SPLIT_DELAY = 5,
     TCFFT_OPT_ROWS_WAVES = 6,
-    TCFFT_OPT_FIRST_CACHED = 7
+    TCFFT_OPT_FIRST_CACHED = 7,
+    TCFFT_OPT_MALL_CHUNK = 8
 };
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
@@ -302,9 +309,11 @@ int tcfft_multi_get_schedule(tcfft_multi plan, int *grid, int *order, int *delay
 int tcfft_multi_autotune(tcfft_multi plan, void *data, void *stream, int *best_grid, int *best_order,
                          float *best_ms);
 
-/* Runs passes [first, first + count) of the plan only (the buffer must hold
- * what the earlier passes produced).  For profiling individual pass kernels;
- * tcfft_exec == tcfft_exec_passes(plan, data, dir, 0, tcfft_plan_num_passes, s). */
+/* Runs passes [first, first + count) of the plan only, each over the whole
+ * batch (the buffer must hold what the earlier passes produced).  For
+ * profiling individual pass kernels.  The result of all passes equals
+ * tcfft_exec's bit for bit; tcfft_exec may run them per batch chunk
+ * (TCFFT_OPT_MALL_CHUNK). */
 int tcfft_plan_num_passes(tcfft_plan plan, int *n_passes);
 int tcfft_exec_passes(tcfft_plan plan, void *data, int direction, int first, int count, void *stream);
 

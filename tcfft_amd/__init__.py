@@ -55,6 +55,7 @@ OPT_COLS_DELAY = 4
 OPT_SPLIT_DELAY = 5
 OPT_ROWS_WAVES = 6
 OPT_FIRST_CACHED = 7
+OPT_MALL_CHUNK = 8
 
 
 class TcfftError(RuntimeError):

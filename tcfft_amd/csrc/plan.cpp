@@ -223,6 +223,9 @@ struct tcfft_plan_s {
     // first pass's output stores of a multi-pass plan: -1 = cached while the
     // buffer is at most 256 MiB (cached_out), 0 = non-temporal, 1 = cached
     int first_cached = -1;
+    // batch chunk of a multi-pass transform larger than the MALL, in MiB
+    // (chunk_signals; 0 = the whole batch at once)
+    int mall_chunk_mib = 256;
     // Split precision at every size (tcfft_plan_prepare_split): the column
     // passes in order, and a workspace of total * 8 bytes (FP32 complex)
     // for the passes whose transposed store cannot go in place.
@@ -452,40 +455,91 @@ tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps) {
 
 // Whether pass i stores its output through the caches: only the first pass of
 // a multi-pass plan (the next pass reads its output back), and by default
-// only while the buffer is at most the 256 MiB of the MALL.  Measured per
-// exec, non-temporal -> cached: 2^20 x 16 (64 MiB) 54.4 -> 46.3 us, 2^21 x 16
-// 101 -> 83.5, 2^22 x 16 (256 MiB) 201 -> 167, 2D 4096 x 4096 84.7 -> 73.5;
-// at 2^24 x 8 (512 MiB) cached stores cost 5% (481 -> 504 us).  A single-pass
-// plan keeps its non-temporal stores (2^16 x 256: 27.7 us, 29.5 cached).
-uint32_t cached_out(const tcfft_plan_s* p, size_t i) {
+// only while the signals it covers (`nsig`, one batch chunk) take at most the
+// 256 MiB of the MALL.  Measured per exec, non-temporal -> cached: 2^20 x 16
+// (64 MiB) 54.4 -> 46.3 us, 2^21 x 16 101 -> 83.5, 2^22 x 16 (256 MiB)
+// 201 -> 167, 2D 4096 x 4096 84.7 -> 73.5; at 2^24 x 8 in one piece
+// (512 MiB) cached stores cost 5% (481 -> 504 us).  A single-pass plan keeps
+// its non-temporal stores (2^16 x 256: 27.7 us, 29.5 cached).
+uint32_t cached_out(const tcfft_plan_s* p, size_t i, uint32_t nsig) {
     if (i != 0 || p->lay.passes.size() < 2 || p->first_cached == 0) return 0u;
-    return p->first_cached == 1 || p->total * 4u <= ((size_t)256 << 20) ? 1u : 0u;
+    const size_t bytes = p->total / (size_t)p->lay.batch * nsig * 4u;
+    return p->first_cached == 1 || bytes <= ((size_t)256 << 20) ? 1u : 0u;
 }
 
-int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t first = 0, size_t count = ~(size_t)0) {
+// Signals per batch chunk.  A multi-pass transform larger than the MALL runs
+// all its passes on one chunk of at most mall_chunk_mib MiB of signals before
+// the next, so that each later pass reads what the pass before has just
+// written from the MALL instead of HBM.  Measured per exec, whole batch ->
+// chunks of 256 MiB: 2^24 x 8 485 -> 426 us, 2^17 x 2048 776 -> 639 us,
+// 2^19 x 2048 3.18 -> 2.64 ms, 2D 256 x 256 x 2048 468 -> 324 us, 2D 4096 x
+// 4096 x 16 1334 -> 925 us (128 and 64 MiB chunks gained less).  Plans with a
+// k_cols_big pass keep the whole batch: 2D 1024 x 1024 x 128 and 512 x 1024
+// x 2048 measured 2% slower in chunks.
+uint32_t chunk_signals(const tcfft_plan_s* p) {
+    const Layout& L = p->lay;
+    const uint32_t B = (uint32_t)L.batch;
+    if (p->mall_chunk_mib == 0 || L.passes.size() < 2 || B < 2) return B;
+    for (const Pass& ps : L.passes)
+        if (ps.kind == 3) return B;
+    const size_t lim = (size_t)p->mall_chunk_mib << 20, per = p->total / B * 4u;
+    if (per * B <= lim || per > lim) return B;
+    const size_t chunks = (per * B + lim - 1) / lim;
+    size_t cs = (B + chunks - 1) / chunks;
+    if (cs * per > lim) cs = lim / per;
+    return (uint32_t)cs;
+}
+
+// The plan's passes [first, first + count) on signals [sig0, sig0 + nsig) of
+// the batch (nsig = 0: the whole batch).  Every pass is uniform over the
+// signals, so a chunk is the same launch with its counts scaled.
+int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t first = 0, size_t count = ~(size_t)0,
+               uint32_t sig0 = 0, uint32_t nsig = 0) {
+    const uint32_t B = (uint32_t)p->lay.batch;
+    if (nsig == 0) nsig = B;
+    buf += p->total / B * sig0;
     const size_t end = count > p->lay.passes.size() - first ? p->lay.passes.size() : first + count;
     for (size_t i = first; i < end; ++i) {
-        const Pass& ps = p->lay.passes[i];
+        Pass ps = p->lay.passes[i];
+        ps.n_tiles = (uint32_t)((uint64_t)ps.n_tiles / B * nsig);
+        ps.outer = (uint32_t)((uint64_t)ps.outer / B * nsig);
         const EpiParams* epi = ps.has_epi ? &ps.epi : nullptr;
         hipError_t e;
         if (ps.kind == 0) {
             tcfft::StartSched sc = rows_sched(p, ps);
-            sc.cached_out = cached_out(p, i);
+            sc.cached_out = cached_out(p, i, nsig);
             e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, sc, rows_wg(p, ps), st);
         } else if (ps.kind == 2) {
             tcfft::StartSched sc = big_sched(p, ps.radix);
-            sc.cached_out = cached_out(p, i);
+            sc.cached_out = cached_out(p, i, nsig);
             e = tcfft::launch_big(ps.radix, buf, ps.n_tiles, sign, p->w4096, ps.big_hi, ps.big_lo, ps.big_M_log2 / 2,
                                   epi, sc, st);
-        }
-        else if (ps.kind == 3)
+        } else if (ps.kind == 3) {
             e = epi ? hipErrorInvalidValue
                     : tcfft::launch_cols_big(ps.radix / 256, buf, ps.s_log2, ps.outer, sign, p->w4096,
                                              tcfft::StartSched{(uint32_t)p->cols_delay, 3u, 0u}, st);
-        else e = tcfft::launch_cols(ps.radix, buf, ps.s_log2, ps.outer, sign, p->w4096, epi, st);
+        } else {
+            e = tcfft::launch_cols(ps.radix, buf, ps.s_log2, ps.outer, sign, p->w4096, epi, st);
+        }
         if (e != hipSuccess) return hip_fail(e);
     }
     return TCFFT_SUCCESS;
+}
+
+// body(sig0, nsig) over the batch chunks of chunk_signals, in order.
+template <class F>
+int for_chunks(const tcfft_plan_s* p, F&& body) {
+    const uint32_t B = (uint32_t)p->lay.batch, cs = chunk_signals(p);
+    for (uint32_t s0 = 0; s0 < B; s0 += cs) {
+        const int rc = body(s0, std::min(cs, B - s0));
+        if (rc) return rc;
+    }
+    return TCFFT_SUCCESS;
+}
+
+// The whole transform (drop-in order, in place), chunked as above.
+int exec_all(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st) {
+    return for_chunks(p, [&](uint32_t s0, uint32_t ns) { return run_passes(p, buf, sign, st, 0, ~(size_t)0, s0, ns); });
 }
 
 tcfft::RevSpec rev_spec(const std::vector<int>& rad) {
@@ -616,9 +670,9 @@ int tcfft_plan_create_1d_tuned(tcfft_plan* plan, int n, int n_batch, int* first_
         rc = tcfft_plan_create_1d_split(&p, n, n_batch, c);
         if (rc) break;
         const int reps = 5;
-        for (int i = 0; i < 2 && !rc; ++i) rc = run_passes(p, (uint32_t*)buf, -1, nullptr);
+        for (int i = 0; i < 2 && !rc; ++i) rc = exec_all(p, (uint32_t*)buf, -1, nullptr);
         if (!rc && hipEventRecord(a, nullptr) != hipSuccess) rc = TCFFT_HIP_ERROR;
-        for (int i = 0; i < reps && !rc; ++i) rc = run_passes(p, (uint32_t*)buf, -1, nullptr);
+        for (int i = 0; i < reps && !rc; ++i) rc = exec_all(p, (uint32_t*)buf, -1, nullptr);
         if (!rc && hipEventRecord(b, nullptr) != hipSuccess) rc = TCFFT_HIP_ERROR;
         if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
         float ms = 0.f;
@@ -742,7 +796,7 @@ int tcfft_exec(tcfft_plan plan, void* data, int direction, void* stream) {
     if (!data || ((uintptr_t)data & 15u)) return TCFFT_INVALID_POINTER;
     if (direction != TCFFT_FORWARD && direction != TCFFT_INVERSE) return TCFFT_INVALID_VALUE;
     DeviceGuard guard(plan->device);
-    return run_passes(plan, (uint32_t*)data, direction, (hipStream_t)stream);
+    return exec_all(plan, (uint32_t*)data, direction, (hipStream_t)stream);
 }
 
 }  // extern "C"
@@ -1055,6 +1109,10 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
         if (value < -1 || value > 1) return TCFFT_INVALID_VALUE;
         plan->first_cached = value;
         return TCFFT_SUCCESS;
+    case TCFFT_OPT_MALL_CHUNK:
+        if (value != 0 && (value < 16 || value > 65536)) return TCFFT_INVALID_VALUE;
+        plan->mall_chunk_mib = value;
+        return TCFFT_SUCCESS;
     default: return TCFFT_INVALID_VALUE;
     }
 }
@@ -1085,12 +1143,12 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
     }
     // mean of five single cold executions after two warm-up ones
     auto time_current = [&](float& ms) {
-        for (int i = 0; i < 2 && !rc; ++i) rc = run_passes(plan, (uint32_t*)data, -1, st);
+        for (int i = 0; i < 2 && !rc; ++i) rc = exec_all(plan, (uint32_t*)data, -1, st);
         float total = 0.f;
         for (int i = 0; i < 5 && !rc; ++i) {
             if (flush && hipMemsetAsync(flush, i + 1, flush_bytes, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
             if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
-            if (!rc) rc = run_passes(plan, (uint32_t*)data, -1, st);
+            if (!rc) rc = exec_all(plan, (uint32_t*)data, -1, st);
             if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
             if (!rc && hipEventSynchronize(b) != hipSuccess) rc = TCFFT_HIP_ERROR;
             float t = 0.f;
@@ -1241,15 +1299,24 @@ int exec_natural_staged(tcfft_plan plan, const void* in, void* out, int directio
             std::vector<int> rx(L0.rad_x.rbegin(), L0.rad_x.rend());
             x_pos = rev_spec(rx);
         }
-        tcfft::StartSched sc = fused_rows ? rows_sched(plan, first) : nat_big_sched(plan, first.radix);
-        sc.cached_out = cached_out(plan, 0);
-        hipError_t e = fused_rows ? tcfft::launch_rows_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
-                                                           plan->w4096, nx_log2, x_pos, sc, st)
-                                  : tcfft::launch_big_nat(first.radix, rsrc, (uint32_t*)out, first.n_tiles, direction,
-                                                          plan->w4096, first.big_hi, first.big_lo,
-                                                          first.big_M_log2 / 2, nx_log2, x_pos, sc, st);
-        if (e != hipSuccess) return hip_fail(e);
-        return run_passes(plan, (uint32_t*)out, direction, st, 1);
+        // per batch chunk (chunk_signals): the fused first pass, then the rest
+        const uint32_t B = (uint32_t)L0.batch;
+        const size_t per = plan->total / B;
+        return for_chunks(plan, [&](uint32_t s0, uint32_t ns) -> int {
+            Pass fp = first;
+            fp.n_tiles = (uint32_t)((uint64_t)first.n_tiles / B * ns);
+            tcfft::StartSched sc = fused_rows ? rows_sched(plan, fp) : nat_big_sched(plan, fp.radix);
+            sc.cached_out = cached_out(plan, 0, ns);
+            const uint32_t* ci = rsrc + per * s0;
+            uint32_t* co = (uint32_t*)out + per * s0;
+            hipError_t e = fused_rows ? tcfft::launch_rows_nat(fp.radix, ci, co, fp.n_tiles, direction, plan->w4096,
+                                                               nx_log2, x_pos, sc, st)
+                                      : tcfft::launch_big_nat(fp.radix, ci, co, fp.n_tiles, direction, plan->w4096,
+                                                              fp.big_hi, fp.big_lo, fp.big_M_log2 / 2, nx_log2, x_pos,
+                                                              sc, st);
+            if (e != hipSuccess) return hip_fail(e);
+            return run_passes(plan, (uint32_t*)out, direction, st, 1, ~(size_t)0, s0, ns);
+        });
     }
     // 1D lines above 65536 points: the LDS-tiled reversal pass, then the
     // plan's passes (every plan takes one of these two routes: 2D rows are
@@ -1263,10 +1330,15 @@ int exec_natural_staged(tcfft_plan plan, const void* in, void* out, int directio
         int rc = stage(src);
         if (rc) return rc;
     }
-    hipError_t e = tcfft::launch_gather_tiled_1d(src, (uint32_t*)out, (uint32_t)ilog2((uint32_t)L.n),
-                                                 (uint32_t)plan->total, rev_spec(L.rad_y), gt, st);
-    if (e != hipSuccess) return hip_fail(e);
-    return run_passes(plan, (uint32_t*)out, direction, st);
+    // per batch chunk (chunk_signals): the reversal, then the passes
+    const tcfft::RevSpec spec = rev_spec(L.rad_y);
+    return for_chunks(plan, [&](uint32_t s0, uint32_t ns) -> int {
+        const size_t off = (size_t)L.n * s0;
+        hipError_t e = tcfft::launch_gather_tiled_1d(src + off, (uint32_t*)out + off, (uint32_t)ilog2((uint32_t)L.n),
+                                                     (uint32_t)((size_t)L.n * ns), spec, gt, st);
+        if (e != hipSuccess) return hip_fail(e);
+        return run_passes(plan, (uint32_t*)out, direction, st, 0, ~(size_t)0, s0, ns);
+    });
 }
 
 }  // namespace tcfft

@@ -139,7 +139,14 @@ __device__ __forceinline__ void split_col_body(const float2* src, float2* dst, u
         for (int r0 = 0; r0 < 16; ++r0)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const f32x2 v = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(p + 256u * (r0 + 16u * j)));
+                // pass 2 reads the intermediate that pass 1 has just stored
+                // through the caches: cached loads; pass 1 streams the input
+                const f32x2* a = reinterpret_cast<const f32x2*>(p + 256u * (r0 + 16u * j));
+                f32x2 v;
+                if constexpr (PASS == 2)
+                    v = *a;
+                else
+                    v = __builtin_nontemporal_load(a);
                 dr[r0][j] = v[0];
                 di[r0][j] = v[1];
             }

@@ -242,3 +242,50 @@ def test_first_pass_cached_bit_identical(shape):
         with pytest.raises(tcfft_amd.TcfftError):
             plan.set_option(tcfft_amd.OPT_FIRST_CACHED, bad)
     plan.destroy()
+
+
+# (kind, n or nx, ny, batch, chunk MiB): ragged last chunks (2^20 x 5 in 3 + 2
+# signals, 2D 256^2 x 300 in five chunks of 60 images), one signal per chunk
+# (2^22 x 3), a 2D shape with a k_cols_big pass (never chunked) and one whose
+# rows are the 65536-point kernel
+CHUNK_CASES = [("1d", 1 << 20, 0, 5, 16), ("1d", 1 << 22, 0, 3, 16), ("2d", 256, 256, 300, 16),
+               ("2d", 1024, 1024, 20, 16), ("2d", 64, 65536, 9, 16), ("1d", 1 << 17, 0, 40, 16)]
+
+
+@pytest.mark.parametrize("case", CHUNK_CASES)
+def test_mall_chunks_bit_identical(case):
+    """Batch chunks (TCFFT_OPT_MALL_CHUNK) give output bit-identical to the
+    whole batch at once, for exec (both directions) and exec_natural (in and
+    out of place); 0 and 16..65536 are accepted, other values rejected."""
+    kind, a, b, batch, mib = case
+    note(f"[gpu] mall chunks {case}")
+    plan = tcfft_amd.Plan1D(a, batch) if kind == "1d" else tcfft_amd.Plan2D(a, b, batch)
+    total = a * (b or 1) * batch
+    rng = np.random.default_rng(total % 1000)
+    x = torch.from_numpy(rng.uniform(-1, 1, (total, 2)).astype(np.float16)).cuda()
+    outs = []
+    for chunk in (0, mib):
+        plan.set_option(tcfft_amd.OPT_MALL_CHUNK, chunk)
+        res = []
+        for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
+            t = x.clone()
+            plan.exec(t.data_ptr(), direction)
+            res.append(t)
+        o = torch.empty_like(x)
+        plan.exec_natural(x.data_ptr(), o.data_ptr())
+        res.append(o)
+        t = x.clone()
+        plan.exec_natural(t.data_ptr(), t.data_ptr(), tcfft_amd.INVERSE)
+        res.append(t)
+        torch.cuda.synchronize()
+        outs.append(res)
+    for i, (u, v) in enumerate(zip(*outs)):
+        assert torch.equal(u, v), (case, i)
+    if kind == "1d" and a <= (1 << 20):
+        h = x.cpu().numpy().astype(np.float64).reshape(batch, a, 2)
+        want = np.fft.fft(h[..., 0] + 1j * h[..., 1], axis=1)
+        assert rel_l2(outs[1][2].view(batch, a, 2), want) <= 1.5e-3
+    for bad in (-1, 1, 15, 65537):
+        with pytest.raises(tcfft_amd.TcfftError):
+            plan.set_option(tcfft_amd.OPT_MALL_CHUNK, bad)
+    plan.destroy()
