@@ -456,11 +456,11 @@ tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps) {
 // Whether pass i stores its output through the caches: only the first pass of
 // a multi-pass plan (the next pass reads its output back), and by default
 // only while the signals it covers (`nsig`, one batch chunk) take at most the
-// 256 MiB of the MALL.  Measured per exec, non-temporal -> cached: 2^20 x 16
-// (64 MiB) 54.4 -> 46.3 us, 2^21 x 16 101 -> 83.5, 2^22 x 16 (256 MiB)
-// 201 -> 167, 2D 4096 x 4096 84.7 -> 73.5; at 2^24 x 8 in one piece
-// (512 MiB) cached stores cost 5% (481 -> 504 us).  A single-pass plan keeps
-// its non-temporal stores (2^16 x 256: 27.7 us, 29.5 cached).
+// 256 MiB of the MALL.  Measured per exec on random input, non-temporal ->
+// cached: 2^20 x 16 (64 MiB) 56.1 -> 51.7 us, 2^21 x 16 105.8 -> 95.1,
+// 2^22 x 16 (256 MiB) 202 -> 186, 2D 4096 x 4096 87.4 -> 77.4; at 2^24 x 8 in
+// one piece (512 MiB) cached stores cost 5% (481 -> 504 us).  A single-pass
+// plan keeps its non-temporal stores (2^16 x 256: 27.7 us, 29.5 cached).
 uint32_t cached_out(const tcfft_plan_s* p, size_t i, uint32_t nsig) {
     if (i != 0 || p->lay.passes.size() < 2 || p->first_cached == 0) return 0u;
     const size_t bytes = p->total / (size_t)p->lay.batch * nsig * 4u;
@@ -470,12 +470,12 @@ uint32_t cached_out(const tcfft_plan_s* p, size_t i, uint32_t nsig) {
 // Signals per batch chunk.  A multi-pass transform larger than the MALL runs
 // all its passes on one chunk of at most mall_chunk_mib MiB of signals before
 // the next, so that each later pass reads what the pass before has just
-// written from the MALL instead of HBM.  Measured per exec, whole batch ->
-// chunks of 256 MiB: 2^24 x 8 485 -> 426 us, 2^17 x 2048 776 -> 639 us,
-// 2^19 x 2048 3.18 -> 2.64 ms, 2D 256 x 256 x 2048 468 -> 324 us, 2D 4096 x
-// 4096 x 16 1334 -> 925 us (128 and 64 MiB chunks gained less).  Plans with a
-// k_cols_big pass keep the whole batch: 2D 1024 x 1024 x 128 and 512 x 1024
-// x 2048 measured 2% slower in chunks.
+// written from the MALL instead of HBM.  Measured per exec on random input,
+// whole batch -> chunks of 256 MiB: 2^24 x 8 491 -> 429 us, 2^17 x 2048
+// 780 -> 716 us, 2^18 x 2048 1571 -> 1442 us, 2D 256 x 256 x 2048 468 -> 338
+// us, 2D 4096 x 4096 x 16 1336 -> 1003 us (128 and 64 MiB chunks gained
+// less).  Plans with a k_cols_big pass keep the whole batch: 2D 1024 x 1024
+// x 128 and 512 x 1024 x 2048 measured 2% slower in chunks.
 uint32_t chunk_signals(const tcfft_plan_s* p) {
     const Layout& L = p->lay;
     const uint32_t B = (uint32_t)L.batch;
