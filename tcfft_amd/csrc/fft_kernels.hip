@@ -20,6 +20,7 @@
 #include "fft_kernels.hpp"
 
 #include <map>
+#include <tuple>
 #include <type_traits>
 #include <mutex>
 
@@ -1159,26 +1160,49 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_c2r_big(
 // `lds` bytes and count the workgroups of `threads` threads resident at once.
 struct KernelInfo {
     hipError_t lds_status;
-    u32 resident;
+    u32 resident;  // workgroups of this shape resident at once
+    u32 cus;       // compute units of the device
 };
 
+// Occupancy of `kernel` launched with `threads` threads and `lds` bytes of
+// dynamic LDS, cached per launch shape (a kernel may run at several
+// workgroup sizes).  The first query of a shape also raises the kernel's
+// dynamic LDS limit to `lds` if it is below (never lowers it: another shape
+// of the same kernel may need more).
 template <class K>
 static KernelInfo kernel_info(K kernel, int threads, size_t lds) {
     static std::mutex mu;
-    static std::map<const void*, KernelInfo> cache;
+    static std::map<std::tuple<const void*, int, size_t>, KernelInfo> cache;
+    static std::map<const void*, size_t> lds_limit;
     std::lock_guard<std::mutex> lock(mu);
-    const auto it = cache.find((const void*)kernel);
+    const auto key = std::make_tuple((const void*)kernel, threads, lds);
+    const auto it = cache.find(key);
     if (it != cache.end()) return it->second;
-    KernelInfo info{hipSuccess, 0};
-    if (lds > 0)
+    KernelInfo info{hipSuccess, 0, 0};
+    size_t& limit = lds_limit[(const void*)kernel];
+    if (lds > limit) {
         info.lds_status = hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (info.lds_status == hipSuccess) limit = lds;
+    }
     int dev = 0, cus = 0, per_cu = 0;
     if (hipGetDevice(&dev) == hipSuccess &&
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kernel, threads, lds) == hipSuccess)
-        info.resident = (u32)(cus * per_cu);
-    cache.emplace((const void*)kernel, info);
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess) {
+        info.cus = (u32)cus;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)kernel, threads, lds) == hipSuccess)
+            info.resident = (u32)(cus * per_cu);
+    }
+    cache.emplace(key, info);
     return info;
+}
+
+// Waves per workgroup of the kernels whose waves each own a 16 KiB LDS slice
+// (k_rows_nat, k_r2c, k_c2r): four, or one when the grid has fewer than four
+// waves per CU, where four-wave workgroups would leave most CUs idle
+// (1024 x 1024 natural order: 64 workgroups on 256 CUs).
+template <class K>
+static u32 slice_wg(K kernel, u32 waves) {
+    const KernelInfo k4 = kernel_info(kernel, 256, 4 * 4096 * sizeof(u32));
+    return waves >= 4u * k4.cus ? 4u : 1u;
 }
 
 template <int T, int SIGN, bool CACHED>
@@ -1246,17 +1270,19 @@ static hipError_t launch_cols_t(u32* buf, u32 s_log2, u32 outer, int sign, const
 template <int T, int SIGN>
 static hipError_t launch_rows_nat_t(const u32* in, u32* out, u32 n_tiles, const float2* w4096, u32 nx_log2,
                                     const RevSpec& x_pos, StartSched sched, hipStream_t st) {
-    constexpr size_t lds = 4 * 4096 * sizeof(u32);  // 16 KiB per wave
     // cached output stores switch only for 4096-point lines (as launch_rows_t)
     auto k = k_rows_nat<T, SIGN, false>;
     if constexpr (T == 16)
         if (sched.cached_out) k = k_rows_nat<T, SIGN, true>;
-    const KernelInfo ki = kernel_info(k, 256, lds);
+    const u32 waves = (n_tiles + 15) / 16;
+    const u32 wg = slice_wg(k, waves);
+    const size_t lds = wg * 4096 * sizeof(u32);  // 16 KiB per wave
+    const KernelInfo ki = kernel_info(k, (int)(64 * wg), lds);
     if (ki.lds_status != hipSuccess) return ki.lds_status;
     if (sched.groups == 0) sched.delay = 0;
     if (sched.delay) sched.resident = ki.resident;
-    const u32 waves = (n_tiles + 15) / 16;
-    hipLaunchKernelGGL(k, dim3((waves + 3) / 4), dim3(256), lds, st, in, out, n_tiles, w4096, nx_log2, x_pos, sched);
+    hipLaunchKernelGGL(k, dim3((waves + wg - 1) / wg), dim3(64 * wg), lds, st, in, out, n_tiles, w4096, nx_log2,
+                       x_pos, sched);
     return hipGetLastError();
 }
 
@@ -1280,12 +1306,13 @@ hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign
 template <int T>
 static hipError_t launch_r2c_t(int sign, const u32* in, u32* out, u32 n_tiles, const float2* w4096,
                                const float2* wn, hipStream_t st) {
-    constexpr size_t lds = 4 * 4096 * sizeof(u32);  // 16 KiB per wave
     auto k = sign < 0 ? k_r2c<T> : k_c2r<T>;
-    const KernelInfo ki = kernel_info(k, 256, lds);
-    if (ki.lds_status != hipSuccess) return ki.lds_status;
     const u32 waves = (n_tiles + 15) / 16;
-    hipLaunchKernelGGL(k, dim3((waves + 3) / 4), dim3(256), lds, st, in, out, n_tiles, w4096, wn);
+    const u32 wg = slice_wg(k, waves);
+    const size_t lds = wg * 4096 * sizeof(u32);  // 16 KiB per wave
+    const KernelInfo ki = kernel_info(k, (int)(64 * wg), lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    hipLaunchKernelGGL(k, dim3((waves + wg - 1) / wg), dim3(64 * wg), lds, st, in, out, n_tiles, w4096, wn);
     return hipGetLastError();
 }
 
