@@ -244,6 +244,11 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * later passes read the earlier ones' output from the MALL.  Plans with a
  * radix-512..2048 column pass (2D nx = 512..2048, 1D 2^25..2^27) and batches
  * whose single signal exceeds the size are not chunked.
+ * TCFFT_OPT_ROWS_TILES (4, 8 or 16; 0, the default, = by the grid): tiles of
+ * 256 points per wave of a contiguous-row first pass of lines up to 2048
+ * points without a folded twiddle (at least the line's tiles; 4096-point
+ * lines always take 16).  The default is the largest that still gives every
+ * SIMD two waves (16 on all but small batches).
  * The schedule never changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,
@@ -253,7 +258,8 @@ enum tcfft_option {
     TCFFT_OPT_SPLIT_DELAY = 5,
     TCFFT_OPT_ROWS_WAVES = 6,
     TCFFT_OPT_FIRST_CACHED = 7,
-    TCFFT_OPT_MALL_CHUNK = 8
+    TCFFT_OPT_MALL_CHUNK = 8,
+    TCFFT_OPT_ROWS_TILES = 9
 };
 int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
@@ -261,8 +267,9 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
  * of its first pass (no stagger, or 3, 5 or 7 groups with delays of 1..4
  * quanta) on `data` (which it overwrites with transforms of itself), keeps
  * the fastest in the plan and reports it; then, with that schedule, the
- * workgroup size of a contiguous-row first pass (TCFFT_OPT_ROWS_WAVES) and
- * the column pass delay (0..4) when the plan has a strided pass of radix
+ * workgroup size of a contiguous-row first pass (TCFFT_OPT_ROWS_WAVES), its
+ * tiles per wave (TCFFT_OPT_ROWS_TILES, lines up to 2048 points) and the
+ * column pass delay (0..4) when the plan has a strided pass of radix
  * 512..2048.  Every timed run
  * follows a write of a 256 MiB buffer that the call allocates and frees
  * (skipped if the allocation fails), so that the data is timed cold, as a

@@ -56,6 +56,7 @@ OPT_SPLIT_DELAY = 5
 OPT_ROWS_WAVES = 6
 OPT_FIRST_CACHED = 7
 OPT_MALL_CHUNK = 8
+OPT_ROWS_TILES = 9
 
 
 class TcfftError(RuntimeError):

@@ -181,18 +181,18 @@ __device__ __forceinline__ void rows_body(TilePtr tile_ptr, u32 ntl, u32 tile0, 
     }
 }
 
-template <int T, bool EPI, int SIGN, bool CACHED = false>
+template <int T, bool EPI, int SIGN, bool CACHED = false, int NT = 16>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows(u32* __restrict__ buf, u32 n_tiles,
                                               const float2* __restrict__ w4096, EpiParams epi,
                                               StartSched sched) {
     const int lane = threadIdx.x & 63;
     // wave index made scalar so that tile addresses are SGPR bases
     const u32 wave = blockIdx.x * (blockDim.x >> 6) + (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const u32 tile0 = wave * 16u;
+    const u32 tile0 = wave * (u32)NT;
     if (tile0 >= n_tiles) return;
     start_delay(sched);
     u32* const base = buf + (size_t)tile0 * 256 + 16 * (lane & 15) + 4 * (lane >> 4);
-    rows_body<T, EPI, SIGN, 16, CACHED>([&](int t) { return base + t * 256; }, min(16u, n_tiles - tile0), tile0,
+    rows_body<T, EPI, SIGN, NT, CACHED>([&](int t) { return base + t * 256; }, min((u32)NT, n_tiles - tile0), tile0,
                                         w4096, epi);
 }
 
@@ -1205,18 +1205,22 @@ static u32 slice_wg(K kernel, u32 waves) {
     return waves >= 4u * k4.cus ? 4u : 1u;
 }
 
-template <int T, int SIGN, bool CACHED>
+template <int T, int SIGN, bool CACHED, int NT = 16>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, const float2* w4096, const EpiParams* epi, StartSched sched,
                                 u32 wg, hipStream_t st) {
-    const u32 waves = (n_tiles + 15) / 16;
+    const u32 waves = (n_tiles + NT - 1) / NT;
     const u32 blocks = (waves + wg - 1) / wg;
     if (sched.groups == 0) sched.delay = 0;
     if (epi) {
-        auto k = k_rows<T, true, SIGN, CACHED>;
-        if (sched.delay) sched.resident = kernel_info(k, (int)(64 * wg), 0).resident;
-        hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * wg), 0, st, buf, n_tiles, w4096, *epi, sched);
+        if constexpr (NT != 16) {
+            return hipErrorInvalidValue;  // the epilogue pass runs 16 tiles per wave only
+        } else {
+            auto k = k_rows<T, true, SIGN, CACHED>;
+            if (sched.delay) sched.resident = kernel_info(k, (int)(64 * wg), 0).resident;
+            hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * wg), 0, st, buf, n_tiles, w4096, *epi, sched);
+        }
     } else {
-        auto k = k_rows<T, false, SIGN, CACHED>;
+        auto k = k_rows<T, false, SIGN, CACHED, NT>;
         if (sched.delay) sched.resident = kernel_info(k, (int)(64 * wg), 0).resident;
         hipLaunchKernelGGL(k, dim3(blocks), dim3(64 * wg), 0, st, buf, n_tiles, w4096, EpiParams{}, sched);
     }
@@ -1224,28 +1228,41 @@ static hipError_t launch_rows_t(u32* buf, u32 n_tiles, const float2* w4096, cons
 }
 
 // Only the 4096-point lines (T = 16) have non-temporal output stores to
-// switch; shorter lines always store through the caches.
+// switch; shorter lines always store through the caches.  Tiles per wave:
+// 16, or 8 / 4 (at least T) for lines of up to 2048 points without an
+// epilogue, which puts more waves on a small grid.
 template <int T>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
-                                StartSched sched, u32 wg, hipStream_t st) {
+                                StartSched sched, u32 wg, u32 tiles, hipStream_t st) {
     if constexpr (T == 16) {
+        if (tiles != 16) return hipErrorInvalidValue;
         if (sched.cached_out)
             return sign < 0 ? launch_rows_t<T, -1, true>(buf, n_tiles, w4096, epi, sched, wg, st)
                             : launch_rows_t<T, +1, true>(buf, n_tiles, w4096, epi, sched, wg, st);
+    } else {
+        if (tiles == 8)
+            return sign < 0 ? launch_rows_t<T, -1, false, 8>(buf, n_tiles, w4096, epi, sched, wg, st)
+                            : launch_rows_t<T, +1, false, 8>(buf, n_tiles, w4096, epi, sched, wg, st);
+        if constexpr (T <= 4) {
+            if (tiles == 4)
+                return sign < 0 ? launch_rows_t<T, -1, false, 4>(buf, n_tiles, w4096, epi, sched, wg, st)
+                                : launch_rows_t<T, +1, false, 4>(buf, n_tiles, w4096, epi, sched, wg, st);
+        }
+        if (tiles != 16) return hipErrorInvalidValue;
     }
     return sign < 0 ? launch_rows_t<T, -1, false>(buf, n_tiles, w4096, epi, sched, wg, st)
                     : launch_rows_t<T, +1, false>(buf, n_tiles, w4096, epi, sched, wg, st);
 }
 
 hipError_t launch_rows(int T, u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
-                       StartSched sched, u32 wg_waves, hipStream_t st) {
+                       StartSched sched, u32 wg_waves, u32 tiles_per_wave, hipStream_t st) {
     if (wg_waves != 1 && wg_waves != 2 && wg_waves != 4) return hipErrorInvalidValue;
     switch (T) {
-    case 1: return launch_rows_t<1>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
-    case 2: return launch_rows_t<2>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
-    case 4: return launch_rows_t<4>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
-    case 8: return launch_rows_t<8>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
-    case 16: return launch_rows_t<16>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, st);
+    case 1: return launch_rows_t<1>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, tiles_per_wave, st);
+    case 2: return launch_rows_t<2>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, tiles_per_wave, st);
+    case 4: return launch_rows_t<4>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, tiles_per_wave, st);
+    case 8: return launch_rows_t<8>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, tiles_per_wave, st);
+    case 16: return launch_rows_t<16>(buf, n_tiles, sign, w4096, epi, sched, wg_waves, tiles_per_wave, st);
     default: return hipErrorInvalidValue;
     }
 }

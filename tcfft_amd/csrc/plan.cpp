@@ -218,6 +218,7 @@ struct tcfft_plan_s {
     // of waves (4096 x 1024: 7.2 us against 8.1 with 4, 1024-point rows of
     // 2D 1024^2 x 8: 14.7 against 15.1)
     int rows_waves = 0;
+    int rows_tiles = 0;  // tiles per wave of a row first pass (rows_nt; 0 = by the grid)
     int cols_delay = 1;   // start delay of the radix-256*W column pass (3 groups; 2D 1024^2 x 8: 36.0 -> 33.8 us)
     int split_delay = 2;  // start delay of the split-precision passes (3 groups; 2^16 x 256: 110.6 -> 106.6 us)
     // first pass's output stores of a multi-pass plan: -1 = cached while the
@@ -447,8 +448,29 @@ uint32_t rows_wg(const tcfft_plan_s* p, const Pass& ps) {
     return ps.radix == 16 && waves >= 32u * (uint32_t)p->cus ? 4u : 1u;
 }
 
+// Tiles (of 256 points) per wave of a contiguous-row first pass: 16 for
+// 4096-point lines and passes with an epilogue; otherwise the plan's setting,
+// or by default the largest of 16, 8, 4 (at least the line's tiles) that
+// still gives every SIMD two waves, so that a small grid spreads over the
+// chip.  Measured (random input; 16 / 8 / 4 tiles): 1024 x 1024 6.17 / 5.23
+// / 4.86 us, 256 x 1024 4.59 / 3.75 / 3.37, 1024 x 4096 9.68 / 9.00 / 10.27,
+// 256 x 16384 9.02 / 8.04 / 8.68; at two waves per SIMD or more, 16 (2D
+// 1024^2 x 8 31.3 us against 34.0 at 8 and 36.0 at 4; 1024 x 8192 14.2
+// against 16.7 and 18.3).
+uint32_t rows_nt(const tcfft_plan_s* p, const Pass& ps) {
+    const uint32_t T = (uint32_t)ps.radix;
+    if (T >= 16 || ps.has_epi) return 16u;
+    const uint32_t lo = T <= 4 ? 4u : 8u;
+    if (p->rows_tiles > 0) return std::max((uint32_t)p->rows_tiles, lo);
+    const uint32_t want = 8u * (uint32_t)p->cus;  // two waves per SIMD
+    for (uint32_t nt = 16u; nt > lo; nt /= 2)
+        if ((ps.n_tiles + nt - 1u) / nt >= want) return nt;
+    return lo;
+}
+
 tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps) {
-    const uint32_t waves = (ps.n_tiles + 15u) / 16u;  // at most one wave per SIMD: a delay of 1
+    const uint32_t nt = rows_nt(p, ps);
+    const uint32_t waves = (ps.n_tiles + nt - 1u) / nt;  // at most one wave per SIMD: a delay of 1
     const int delay = p->big_delay >= 0 ? p->big_delay : (waves <= 4u * (uint32_t)p->cus ? 1 : 0);
     return tcfft::StartSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
 }
@@ -508,7 +530,8 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
         if (ps.kind == 0) {
             tcfft::StartSched sc = rows_sched(p, ps);
             sc.cached_out = cached_out(p, i, nsig);
-            e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, sc, rows_wg(p, ps), st);
+            e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, sc, rows_wg(p, ps), rows_nt(p, ps),
+                                   st);
         } else if (ps.kind == 2) {
             tcfft::StartSched sc = big_sched(p, ps.radix);
             sc.cached_out = cached_out(p, i, nsig);
@@ -1109,6 +1132,10 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value) {
         if (value < -1 || value > 1) return TCFFT_INVALID_VALUE;
         plan->first_cached = value;
         return TCFFT_SUCCESS;
+    case TCFFT_OPT_ROWS_TILES:
+        if (value != 0 && value != 4 && value != 8 && value != 16) return TCFFT_INVALID_VALUE;
+        plan->rows_tiles = value;
+        return TCFFT_SUCCESS;
     case TCFFT_OPT_MALL_CHUNK:
         if (value != 0 && (value < 16 || value > 65536)) return TCFFT_INVALID_VALUE;
         plan->mall_chunk_mib = value;
@@ -1209,6 +1236,16 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
         tune(wg, [&](const Cand& c) { plan->rows_waves = c.a; });
         plan->rows_waves = rc ? saved : wg[0].a;
         if (!rc) best_t = std::min(best_t, wg[0].ms);
+    }
+    // then the tiles per wave of such a pass of lines up to 2048 points
+    const Pass& p0 = plan->lay.passes[0];
+    if (!rc && p0.kind == 0 && p0.radix < 16 && !p0.has_epi) {
+        const int saved = plan->rows_tiles;
+        std::vector<Cand> tiles{{16, 0, 0.f}, {8, 0, 0.f}};
+        if (p0.radix <= 4) tiles.push_back({4, 0, 0.f});
+        tune(tiles, [&](const Cand& c) { plan->rows_tiles = c.a; });
+        plan->rows_tiles = rc ? saved : tiles[0].a;
+        if (!rc) best_t = std::min(best_t, tiles[0].ms);
     }
     // then the start delay of a radix-256*W column pass, if the plan has one
     bool has_cols_big = false;

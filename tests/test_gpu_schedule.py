@@ -289,3 +289,48 @@ def test_mall_chunks_bit_identical(case):
         with pytest.raises(tcfft_amd.TcfftError):
             plan.set_option(tcfft_amd.OPT_MALL_CHUNK, bad)
     plan.destroy()
+
+
+@pytest.mark.parametrize("n,batch", [(1024, 1024), (256, 300), (2048, 64), (512, 4096), (1 << 18, 3)])
+def test_rows_tiles_per_wave_bit_identical(n, batch):
+    """Tiles per wave of a contiguous-row first pass (TCFFT_OPT_ROWS_TILES 0 =
+    by the grid, 4, 8, 16; 2048-point lines take at least 8; a first pass with
+    a folded twiddle, 2^18 split at 1024 points, always 16) never change
+    results, with any workgroup size, and after tcfft_plan_autotune; other
+    values are rejected."""
+    note(f"[gpu] rows tiles per wave n={n} batch={batch}")
+    x, d = dropin_input(n, batch, n + 7 * batch)
+    plan = tcfft_amd.Plan1D(n, batch, first_log2=10) if n > 65536 else tcfft_amd.Plan1D(n, batch)
+    if n > 65536:
+        xr = x[:, tcfft_amd.gen_rev(n, plan.radices)]
+        d = torch.from_numpy(np.ascontiguousarray(np.stack([xr.real, xr.imag], -1).astype(np.float16))).cuda()
+    ref = d.clone()
+    plan.exec(ref.data_ptr())
+    torch.cuda.synchronize()
+    assert rel_l2(ref, np.fft.fft(x, axis=1)) <= (1e-3 if n <= 65536 else 1.5e-3)
+    for tiles in (16, 8, 4, 0):
+        plan.set_option(tcfft_amd.OPT_ROWS_TILES, tiles)
+        for wg in (1, 4):
+            plan.set_option(tcfft_amd.OPT_ROWS_WAVES, wg)
+            for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
+                t = d.clone()
+                plan.exec(t.data_ptr(), direction)
+                torch.cuda.synchronize()
+                if direction == tcfft_amd.FORWARD:
+                    assert torch.equal(t, ref), (tiles, wg)
+                else:
+                    if tiles == 16 and wg == 1:
+                        inv_ref = t
+                    assert torch.equal(t, inv_ref), (tiles, wg)
+    plan.set_option(tcfft_amd.OPT_ROWS_WAVES, 0)
+    plan.set_option(tcfft_amd.OPT_ROWS_TILES, 0)
+    scratch = d.clone()
+    plan.autotune(scratch.data_ptr())
+    t = d.clone()
+    plan.exec(t.data_ptr())
+    torch.cuda.synchronize()
+    assert torch.equal(t, ref)
+    for bad in (-1, 2, 12, 32):
+        with pytest.raises(tcfft_amd.TcfftError):
+            plan.set_option(tcfft_amd.OPT_ROWS_TILES, bad)
+    plan.destroy()
