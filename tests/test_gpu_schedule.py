@@ -281,6 +281,13 @@ def test_mall_chunks_bit_identical(case):
         outs.append(res)
     for i, (u, v) in enumerate(zip(*outs)):
         assert torch.equal(u, v), (case, i)
+    # tcfft_exec_passes runs each pass over the whole batch: pass by pass it
+    # equals the chunked exec bit for bit
+    t = x.clone()
+    for p in range(plan.num_passes):
+        plan.exec_passes(t.data_ptr(), p, 1)
+    torch.cuda.synchronize()
+    assert torch.equal(t, outs[1][0]), case
     if kind == "1d" and a <= (1 << 20):
         h = x.cpu().numpy().astype(np.float64).reshape(batch, a, 2)
         want = np.fft.fft(h[..., 0] + 1j * h[..., 1], axis=1)
