@@ -877,6 +877,151 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_cols_big(u32* __re
 }
 
 // ----------------------------------------------------------------------------
+// k_cols_bigw<W>: k_cols_big<W> for 32 adjacent columns per workgroup of 2W
+// waves, with 16-byte global accesses (used for W = 2; at W = 4 the 144 KiB
+// tile leaves one workgroup per CU, and it measured slower than k_cols_big:
+// 2D 1024^2 x 8 32.3 -> 34.5 us, 2^26 x 4 972 -> 1098 us).  The workgroup loads
+// the block's 256W rows x 128 bytes into an LDS tile (16-byte lanes, 16W rows
+// per round); wave (w, q) = (row group, 16-column group) reads the 256 rows
+// j2 = w of its 16 columns in k_cols_big's B-form layout and runs the same two
+// stages; after a barrier the tile region holds the radix-W exchange
+// ([q][source wave][tile][lane][4 words], one round), each wave's radix-W
+// outputs stay in registers until a second barrier, go back into the tile and
+// leave with 16-byte non-temporal stores of whole 128-byte row pieces.  Tile
+// word of (row r, column c): 36r + (c ^ 16*((r >> 6) & 1)) (rows of 144
+// bytes): the B-form column reads (rows 4g apart) fall on four different
+// 16-bank groups, the output writes (rows 64g apart) on two.
+template <int W>
+struct ColsBigWCfg {
+    static constexpr u32 ROWS = 256u * W;
+    static constexpr u32 PITCH = 36u;
+    static constexpr u32 TILE_WORDS = ROWS * PITCH;
+    static constexpr u32 XCH_WORDS = 2u * W * 16u * 256u;  // [q][src][tile][lane][4]
+    static constexpr u32 LDS_WORDS = TILE_WORDS > XCH_WORDS ? TILE_WORDS : XCH_WORDS;
+};
+__device__ __forceinline__ u32 colsbw_word(u32 r, u32 c) { return 36u * r + (c ^ (((r >> 6) & 1u) << 4)); }
+
+template <int W, int SIGN>
+__global__ __launch_bounds__(128 * W) TCFFT_MFMA_BUDGET void k_cols_bigw(u32* __restrict__ buf, u32 s_log2,
+                                                                       u32 n_groups,
+                                                                       const float2* __restrict__ w4096,
+                                                                       StartSched sched) {
+    static_assert(W == 2 || W == 4, "k_cols_bigw: W = 2, 4 (the tile of W = 8 exceeds the LDS)");
+    using C = ColsBigWCfg<W>;
+    extern __shared__ u32 lds[];
+    const u32 grp = blockIdx.x;
+    if (grp >= n_groups) return;
+    start_delay(sched);
+    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+    const u32 tid = threadIdx.x;
+    const u32 wv = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 w = wv % W, q = wv / W;
+    const u32 cg_log2 = s_log2 - 5u;  // 32-column groups per block
+    const u32 blk = grp >> cg_log2;
+    const u32 col0 = (grp & ((1u << cg_log2) - 1u)) * 32u;
+    u32* const base = buf + (((size_t)blk * C::ROWS) << s_log2) + col0;
+    // load phase: thread (rr = tid >> 3, cq = tid & 7) takes rows rr + 16W*k,
+    // columns 4cq..4cq+3
+    const u32 rr = tid >> 3, cq = tid & 7u;
+
+    // table values before the data (global loads complete in issue order)
+    float2 t_step[4], t_rec0[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const u32 k0 = 4u * g + i;
+        t_step[i] = w4096[16u * k0];                    // W_256^{k0}
+        t_rec0[i] = w4096[(k0 * w * (16u / W)) & 4095u];  // W_{256W}^{k0*w}
+    }
+    const float2 t_s1 = w4096[((u32)c16 * w * (256u / W)) & 4095u];  // W_{256W}^{16*c16*w}
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        u32 v[16][4];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) load4_nt(v[k], base + ((size_t)(rr + 16u * W * k) << s_log2) + 4u * cq);
+#pragma unroll
+        for (int k = 0; k < 16; ++k) store4(lds + colsbw_word(rr + 16u * W * k, 4u * cq), v[k]);
+    }
+    __syncthreads();
+    u32 d[16][4];
+#pragma unroll
+    for (int t = 0; t < 16; ++t)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[t][i] = lds[colsbw_word(4u * g + i + 16u * t + 256u * w, 16u * q + c16)];
+
+    const Dft16Op op = make_dft16(lane, SIGN);
+    {
+        cf step[4], rec[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            step[i] = cf{t_step[i].x, sg * t_step[i].y};
+            rec[i] = cf{t_rec0[i].x, sg * t_rec0[i].y};
+        }
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+            f32x4 re, im;
+            dft16_b(d[t], op, re, im);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                d[t][i] = pack(cmul(cf{re[i], im[i]}, rec[i]));
+                if (t < 15) rec[i] = cmul(rec[i], step[i]);
+            }
+        }
+    }
+    u32 e[16][4];
+    transpose_gi_tile(d, e);  // tile = k0, (g,i) = j1
+    const Dft16Op op1 = make_dft16_scaled(lane_id(), SIGN, cf{t_s1.x, sg * t_s1.y});
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+        f32x4 re, im;
+        dft16_b(e[t], op1, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) e[t][i] = pack(cf{re[i], im[i]});
+    }
+    __syncthreads();  // every wave has read its input columns: the tile region is free
+    u32* const xq = lds + q * (W * 16u * 256u);
+#pragma unroll
+    for (int t = 0; t < 16; ++t) store4(xq + ((w * 16u + (u32)t) * 64u + (u32)lane) * 4u, e[t]);
+    __syncthreads();
+    // radix-W over the sources for tiles k0 = w*TPW + ti; outputs row
+    // k0 + 16 i + 64 g + 256 k2, column 16 q + c16
+    constexpr u32 TPW = 16u / W;
+    u32 o[TPW][4][W];
+#pragma unroll
+    for (u32 ti = 0; ti < TPW; ++ti) {
+        const u32 k0 = w * TPW + ti;
+        u32 v[W][4];
+#pragma unroll
+        for (int src = 0; src < W; ++src) load4(v[src], xq + (((u32)src * 16u + k0) * 64u + (u32)lane) * 4u);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            cf x[W];
+#pragma unroll
+            for (int src = 0; src < W; ++src) x[src] = unpack(v[src][i]);
+            small_dft<W>(x, SIGN);
+#pragma unroll
+            for (int k2 = 0; k2 < W; ++k2) o[ti][i][k2] = pack(x[k2]);
+        }
+    }
+    __syncthreads();  // the exchange has been read: the tile region takes the outputs
+#pragma unroll
+    for (u32 ti = 0; ti < TPW; ++ti)
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int k2 = 0; k2 < W; ++k2)
+                lds[colsbw_word(w * TPW + ti + 16u * i + 64u * g + 256u * k2, 16u * q + c16)] = o[ti][i][k2];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        u32 v[4];
+        load4(v, lds + colsbw_word(rr + 16u * W * k, 4u * cq));
+        store4_nt(base + ((size_t)(rr + 16u * W * k) << s_log2) + 4u * cq, v);
+    }
+}
+
+// ----------------------------------------------------------------------------
 // k_cols_small<R>: strided radix-R pass (R = 2, 4, 8), FP32 VALU, four adjacent
 // columns per lane (16-byte loads).  Rows are in natural order within the pass.
 template <int R, bool EPI>
@@ -1518,6 +1663,19 @@ hipError_t launch_big_nat(int W, const u32* in, u32* out, u32 n_signals, int sig
 template <int W, int SIGN>
 static hipError_t launch_cols_big_t(u32* buf, u32 s_log2, u32 outer, const float2* w4096, StartSched sched,
                                     hipStream_t st) {
+    if constexpr (W == 2) {
+        if (s_log2 >= 5u) {  // 32 columns per workgroup, 16-byte accesses
+            const size_t lds = ColsBigWCfg<W>::LDS_WORDS * sizeof(u32);
+            auto k = k_cols_bigw<W, SIGN>;
+            const KernelInfo ki = kernel_info(k, 128 * W, lds);
+            if (ki.lds_status != hipSuccess) return ki.lds_status;
+            if (sched.groups == 0) sched.delay = 0;
+            if (sched.delay) sched.resident = ki.resident;
+            const u32 n_groups = outer << (s_log2 - 5u);
+            hipLaunchKernelGGL(k, dim3(n_groups), dim3(128 * W), lds, st, buf, s_log2, n_groups, w4096, sched);
+            return hipGetLastError();
+        }
+    }
     const size_t lds = ColsBigCfg<W>::LDS_WORDS * sizeof(u32);
     auto k = k_cols_big<W, SIGN>;
     const KernelInfo ki = kernel_info(k, 64 * W, lds);
