@@ -244,12 +244,15 @@ def test_first_pass_cached_bit_identical(shape):
     plan.destroy()
 
 
-# (kind, n or nx, ny, batch, chunk MiB): ragged last chunks (2^20 x 5 in 3 + 2
-# signals, 2D 256^2 x 300 in five chunks of 60 images), one signal per chunk
-# (2^22 x 3), a 2D shape with a k_cols_big pass (never chunked) and one whose
-# rows are the 65536-point kernel
+# (kind, n or nx, ny, batch, chunk MiB[, first pass log2]): ragged last chunks
+# (2^20 x 5 in 3 + 2 signals, 2D 256^2 x 300 in five chunks of 60 images), one
+# signal per chunk (2^22 x 3), a 2D shape with a k_cols_big pass (never
+# chunked), one whose rows are the 65536-point kernel, and three-pass plans
+# whose middle column pass folds the next pass's twiddle (2D 65536 x 256: two
+# radix-256 column passes; 1D 2^24 split at 4096 points), one signal per chunk
 CHUNK_CASES = [("1d", 1 << 20, 0, 5, 16), ("1d", 1 << 22, 0, 3, 16), ("2d", 256, 256, 300, 16),
-               ("2d", 1024, 1024, 20, 16), ("2d", 64, 65536, 9, 16), ("1d", 1 << 17, 0, 40, 16)]
+               ("2d", 1024, 1024, 20, 16), ("2d", 64, 65536, 9, 16), ("1d", 1 << 17, 0, 40, 16),
+               ("2d", 65536, 256, 2, 64), ("1d", 1 << 24, 0, 3, 64, 12)]
 
 
 @pytest.mark.parametrize("case", CHUNK_CASES)
@@ -257,9 +260,14 @@ def test_mall_chunks_bit_identical(case):
     """Batch chunks (TCFFT_OPT_MALL_CHUNK) give output bit-identical to the
     whole batch at once, for exec (both directions) and exec_natural (in and
     out of place); 0 and 16..65536 are accepted, other values rejected."""
-    kind, a, b, batch, mib = case
+    kind, a, b, batch, mib = case[:5]
     note(f"[gpu] mall chunks {case}")
-    plan = tcfft_amd.Plan1D(a, batch) if kind == "1d" else tcfft_amd.Plan2D(a, b, batch)
+    if kind == "1d":
+        plan = tcfft_amd.Plan1D(a, batch, first_log2=case[5]) if len(case) > 5 else tcfft_amd.Plan1D(a, batch)
+    else:
+        plan = tcfft_amd.Plan2D(a, b, batch)
+    if len(case) > 5 or a == 65536:
+        assert plan.num_passes == 3, plan.num_passes
     total = a * (b or 1) * batch
     rng = np.random.default_rng(total % 1000)
     x = torch.from_numpy(rng.uniform(-1, 1, (total, 2)).astype(np.float16)).cuda()
