@@ -268,7 +268,8 @@ __device__ __forceinline__ void nat_rows_load(u32 (&d)[16][4], const u32* __rest
 template <int T, int SIGN, bool CACHED = false>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* in, u32* out,
                                                                   u32 n_tiles, const float2* __restrict__ w4096,
-                                                                  u32 nx_log2, RevSpec x_pos, StartSched sched) {
+                                                                  u32 nx_log2, const u32* __restrict__ xpos,
+                                                                  StartSched sched) {
     extern __shared__ u32 lds[];
     constexpr u32 L = 256u * T;
     const int lane = threadIdx.x & 63;
@@ -282,15 +283,21 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* i
     const int c16 = lane & 15, g = lane >> 4;
     auto load = [&](u32 (&d)[16][4]) { nat_rows_load<T>(d, in + (size_t)tile0 * 256u, wl, ntl, lane); };
     const u32 lane_off = 16u * c16 + 4u * g;
-    auto tile_ptr = [&](int t) -> u32* {
-        const u32 line = (tile0 + (u32)t) / T;
+    // destination line of each of the wave's 16/T lines, once per line and
+    // wave-uniform (2D: the x position from the plan's table)
+    constexpr u32 LPW = 16u / T;
+    u32 dst_line[LPW];
+#pragma unroll
+    for (u32 l = 0; l < LPW; ++l) {
+        const u32 line = tile0 / T + l;
         u32 dst = line;
         if (nx_log2) {
-            const u32 x = line & ((1u << nx_log2) - 1u);
-            dst = (line & ~((1u << nx_log2) - 1u)) | rev_pos_pow2(x, x_pos);
+            const u32 mask = (1u << nx_log2) - 1u;
+            dst = (line & ~mask) | xpos[line & mask];
         }
-        return out + (size_t)dst * L + 256u * ((u32)t % T) + lane_off;
-    };
+        dst_line[l] = (u32)__builtin_amdgcn_readfirstlane((int)dst);
+    }
+    auto tile_ptr = [&](int t) -> u32* { return out + (size_t)dst_line[t / T] * L + 256u * ((u32)t % T) + lane_off; };
     rows_body<T, false, SIGN, 16, CACHED>(tile_ptr, ntl, tile0, w4096, EpiParams{}, load);
 }
 
@@ -1531,7 +1538,7 @@ static hipError_t launch_cols_t(u32* buf, u32 s_log2, u32 outer, int sign, const
 
 template <int T, int SIGN>
 static hipError_t launch_rows_nat_t(const u32* in, u32* out, u32 n_tiles, const float2* w4096, u32 nx_log2,
-                                    const RevSpec& x_pos, StartSched sched, hipStream_t st) {
+                                    const u32* xpos, StartSched sched, hipStream_t st) {
     // cached output stores switch only for 4096-point lines (as launch_rows_t)
     auto k = k_rows_nat<T, SIGN, false>;
     if constexpr (T == 16)
@@ -1544,23 +1551,24 @@ static hipError_t launch_rows_nat_t(const u32* in, u32* out, u32 n_tiles, const 
     if (sched.groups == 0) sched.delay = 0;
     if (sched.delay) sched.resident = ki.resident;
     hipLaunchKernelGGL(k, dim3((waves + wg - 1) / wg), dim3(64 * wg), lds, st, in, out, n_tiles, w4096, nx_log2,
-                       x_pos, sched);
+                       xpos, sched);
     return hipGetLastError();
 }
 
 hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign, const float2* w4096, u32 nx_log2,
-                           const RevSpec& x_pos, StartSched sched, hipStream_t st) {
+                           const u32* xpos, StartSched sched, hipStream_t st) {
+    if (nx_log2 && !xpos) return hipErrorInvalidValue;
     switch (T * 2 + (sign < 0 ? 0 : 1)) {
-    case 2: return launch_rows_nat_t<1, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 3: return launch_rows_nat_t<1, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 4: return launch_rows_nat_t<2, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 5: return launch_rows_nat_t<2, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 8: return launch_rows_nat_t<4, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 9: return launch_rows_nat_t<4, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 16: return launch_rows_nat_t<8, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 17: return launch_rows_nat_t<8, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 32: return launch_rows_nat_t<16, -1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
-    case 33: return launch_rows_nat_t<16, +1>(in, out, n_tiles, w4096, nx_log2, x_pos, sched, st);
+    case 2: return launch_rows_nat_t<1, -1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 3: return launch_rows_nat_t<1, +1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 4: return launch_rows_nat_t<2, -1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 5: return launch_rows_nat_t<2, +1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 8: return launch_rows_nat_t<4, -1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 9: return launch_rows_nat_t<4, +1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 16: return launch_rows_nat_t<8, -1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 17: return launch_rows_nat_t<8, +1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 32: return launch_rows_nat_t<16, -1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    case 33: return launch_rows_nat_t<16, +1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
     default: return hipErrorInvalidValue;
     }
 }

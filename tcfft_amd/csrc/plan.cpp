@@ -205,6 +205,9 @@ struct tcfft_plan_s {
     // allocates.  One per plan: in-place natural-order calls on one plan
     // must not overlap.
     void* workspace = nullptr;
+    // 2D: output line of input row x for the natural-order row pass (nx
+    // words, the digit reversal over the x radices in reverse order)
+    uint32_t* xpos = nullptr;
     size_t workspace_bytes = 0;
     size_t total = 0;  // elements in the buffer
     // signals per pass pair in tcfft_exec_split, 0 = the whole batch (2^16 x
@@ -272,6 +275,36 @@ int upload(tcfft_plan_s* p, const std::vector<float2>& h, float2** dst) {
     e = hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e);
     *dst = (float2*)d;
+    return TCFFT_SUCCESS;
+}
+
+// Position of input row x (0 <= x < nx) among the rows of a 2D natural-order
+// row pass's output: the digits of x taken from the low end by the x radices
+// in reverse order and reassembled most significant first (the digit
+// reversal the column passes consume, as gen_rev, tcfft_doit_half_2d.cpp:40-50).
+uint32_t xpos_of(uint32_t x, const std::vector<int>& rad_x) {
+    uint32_t idx = 0, rest = x;
+    std::vector<uint32_t> digits;
+    for (auto it = rad_x.rbegin(); it != rad_x.rend(); ++it) {
+        digits.push_back(rest % (uint32_t)*it);
+        rest /= (uint32_t)*it;
+    }
+    size_t i = 0;
+    for (auto it = rad_x.rbegin(); it != rad_x.rend(); ++it, ++i) idx = idx * (uint32_t)*it + digits[i];
+    return idx;
+}
+
+int upload_xpos(tcfft_plan_s* p) {
+    const uint32_t nx = (uint32_t)p->lay.nx;
+    std::vector<uint32_t> h(nx);
+    for (uint32_t x = 0; x < nx; ++x) h[x] = xpos_of(x, p->lay.rad_x);
+    void* d = nullptr;
+    hipError_t e = hipMalloc(&d, nx * sizeof(uint32_t));
+    if (e != hipSuccess) return hip_fail(e);
+    p->allocs.push_back(d);
+    e = hipMemcpy(d, h.data(), nx * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e);
+    p->xpos = (uint32_t*)d;
     return TCFFT_SUCCESS;
 }
 
@@ -393,6 +426,7 @@ int create_common(tcfft_plan* plan, const Layout& lay, bool stage_ws = true) {
     p->lay = lay;
     p->total = lay.dims == 1 ? (size_t)lay.n * lay.batch : (size_t)lay.nx * lay.ny * lay.batch;
     int rc = finish_plan(p);
+    if (!rc && lay.dims == 2) rc = upload_xpos(p);
     if (!rc && stage_ws && natural_needs_stage(lay)) {
         const hipError_t e = hipMalloc(&p->workspace, p->total * 4u);
         if (e != hipSuccess) {
@@ -1347,7 +1381,7 @@ int exec_natural_staged(tcfft_plan plan, const void* in, void* out, int directio
             const uint32_t* ci = rsrc + per * s0;
             uint32_t* co = (uint32_t*)out + per * s0;
             hipError_t e = fused_rows ? tcfft::launch_rows_nat(fp.radix, ci, co, fp.n_tiles, direction, plan->w4096,
-                                                               nx_log2, x_pos, sc, st)
+                                                               nx_log2, plan->xpos, sc, st)
                                       : tcfft::launch_big_nat(fp.radix, ci, co, fp.n_tiles, direction, plan->w4096,
                                                               fp.big_hi, fp.big_lo, fp.big_M_log2 / 2, nx_log2, x_pos,
                                                               sc, st);
