@@ -235,16 +235,27 @@ __device__ __forceinline__ u32 rev_pos_pow2(u32 x, const RevSpec& s) {
 
 // Reads a wave's natural-order chunk (ntl tiles, whole lines of 256*T) from
 // its LDS slice into the digit-reversed register layout of k_rows.
+// The word of (t, j), nat_lds_word(m) with m = (t/T) L + (t%T) + T c16 +
+// 16T (4g + j), is (t/T) L + ((t%T) op y_j) with a per-lane y_j (the swizzle
+// depends on bits 6-7 of m, which t does not reach): op is + for T <= 4 (y_j
+// has no bits below 2) and ^ for T = 8, 16, so the per-element address is a
+// constant offset or one XOR.
 template <int T>
 __device__ __forceinline__ void nat_rows_read(u32 (&d)[16][4], const u32* wl, u32 ntl, int lane) {
     constexpr u32 L = 256u * T;
-    const int c16 = lane & 15, g = lane >> 4;
+    const u32 c16 = (u32)lane & 15u, g = (u32)lane >> 4;
+    u32 y[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const u32 lp = T * c16 + 16u * T * (4u * g + (u32)j);
+        y[j] = lp ^ (((lp >> 6) & 3u) * 0x14u);
+    }
 #pragma unroll
     for (int t = 0; t < 16; ++t)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            const u32 m = (u32)(t / T) * L + (u32)(t % T) + T * (u32)c16 + 16u * T * (4u * g + j);
-            d[t][j] = (u32)t < ntl ? wl[nat_lds_word(m)] : 0u;
+            const u32 off = T <= 4 ? (u32)(t % T) + y[j] : (y[j] ^ (u32)(t % T));
+            d[t][j] = (u32)t < ntl ? wl[(u32)(t / T) * L + off] : 0u;
         }
 }
 
@@ -1161,11 +1172,20 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
             for (int t = 0; t < 16; ++t) store4(lds + nat_big_word<W>(4096u * w + 256u * t + 4u * lane), v[t]);
         }
         __syncthreads();
+        // reader word of register (t, j): nat_big_word<W>(w + W (t + 16 c16 +
+        // 256 (4g + j))) = lane_base + ((w + W t) ^ xlo) + 256 W j: the swizzle
+        // bits come from the lane's part q = 16W c16 + 1024W g alone, its bits
+        // below log2(16W) (xlo) meet w + W t, the rest (q's) do not
+        constexpr u32 LW = W == 2 ? 5u : W == 4 ? 6u : 7u;  // log2(16W)
+        const u32 q = 16u * W * (u32)c16 + 1024u * W * (u32)g;
+        const u32 x = (((q >> NatBigSwz<W>::A) & 3u) << 2) ^ (((q >> NatBigSwz<W>::B) & 3u) << 4);
+        const u32 xlo = x & ((1u << LW) - 1u), lane_base = q ^ (x & ~((1u << LW) - 1u));
 #pragma unroll
-        for (int t = 0; t < 16; ++t)
+        for (int t = 0; t < 16; ++t) {
+            const u32* a = lds + lane_base + ((w + W * (u32)t) ^ xlo);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                d[t][j] = lds[nat_big_word<W>(w + W * ((u32)t + 16u * c16 + 256u * (4u * g + j)))];
+            for (int j = 0; j < 4; ++j) d[t][j] = a[256u * W * j];
+        }
     } else {
         // data first, the twiddle tables after the exchange: the pending
         // half of the loaded data and the reader registers fit the budget
@@ -1173,6 +1193,10 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
         const u32* src = in + (size_t)sig * N + 4096u * w + 4u * lane;
 #pragma unroll
         for (int t = 0; t < 16; ++t) load4_nt(v[t], src + 256u * t);
+        // reader word of register (t, j): nat16_word(w + 16 (t + 16 c16 +
+        // 256 (4g + j))) = lane_base + 16 (t ^ g) + 2048 j, computed once per
+        // lane (the per-element form cost about 1600 VALU instructions a wave)
+        const u32 lane_base = ((u32)w ^ (4u * ((u32)c16 & 3u))) + 256u * ((u32)c16 & 7u) + 8192u * (u32)g;
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
             if (r) __syncthreads();  // round 0's reads are done
@@ -1182,10 +1206,11 @@ __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
             __syncthreads();
             if ((u32)(c16 >> 3) == (u32)r) {
 #pragma unroll
-                for (int t = 0; t < 16; ++t)
+                for (int t = 0; t < 16; ++t) {
+                    const u32* a = lds + lane_base + 16u * ((u32)t ^ (u32)g);
 #pragma unroll
-                    for (int j = 0; j < 4; ++j)
-                        d[t][j] = lds[nat16_word(w + 16u * ((u32)t + 16u * c16 + 256u * (4u * g + j)))];
+                    for (int j = 0; j < 4; ++j) d[t][j] = a[2048u * j];
+                }
             }
         }
         big_tw_load<W, false>(bt, lane_id(), (int)w, sig, w4096, wn_hi, wn_lo, wn_lo_bits, EpiParams{});
