@@ -240,8 +240,8 @@ __device__ __forceinline__ u32 rev_pos_pow2(u32 x, const RevSpec& s) {
 // depends on bits 6-7 of m, which t does not reach): op is + for T <= 4 (y_j
 // has no bits below 2) and ^ for T = 8, 16, so the per-element address is a
 // constant offset or one XOR.
-template <int T>
-__device__ __forceinline__ void nat_rows_read(u32 (&d)[16][4], const u32* wl, u32 ntl, int lane) {
+template <int T, int NT = 16>
+__device__ __forceinline__ void nat_rows_read(u32 (&d)[NT][4], const u32* wl, u32 ntl, int lane) {
     constexpr u32 L = 256u * T;
     const u32 c16 = (u32)lane & 15u, g = (u32)lane >> 4;
     u32 y[4];
@@ -251,7 +251,7 @@ __device__ __forceinline__ void nat_rows_read(u32 (&d)[16][4], const u32* wl, u3
         y[j] = lp ^ (((lp >> 6) & 3u) * 0x14u);
     }
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
+    for (int t = 0; t < NT; ++t)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             const u32 off = T <= 4 ? (u32)(t % T) + y[j] : (y[j] ^ (u32)(t % T));
@@ -1145,6 +1145,17 @@ __device__ __forceinline__ u32 nat_split_word(u32 n) {
     return m ^ (((m >> 8) & 3u) << 2) ^ (((m >> 13) & 3u) << 4);
 }
 
+// nat_split_word<B>(p + q) for p < 2^LW and a lane index q whose bits below
+// LW are zero (LW <= 8 <= B): the swizzle bits come from q alone, so the
+// word is base + (p ^ xlo) with base and xlo computed once per lane.
+template <int B, int LW>
+__device__ __forceinline__ void nat_split_lane(u32 q, u32& base, u32& xlo) {
+    const u32 m = (q & ((1u << B) - 1u)) | ((q >> (B + 1)) << B);
+    const u32 x = (((m >> 8) & 3u) << 2) ^ (((m >> 13) & 3u) << 4);
+    xlo = x & ((1u << LW) - 1u);
+    base = m ^ (x & ~((1u << LW) - 1u));
+}
+
 template <int W, int SIGN, bool CACHED = false>
 __global__ __launch_bounds__(64 * W) TCFFT_MFMA_BUDGET void k_big_nat(
     const u32* in, u32* out, const float2* __restrict__ w4096,
@@ -1897,19 +1908,6 @@ __global__ __launch_bounds__(1024) void k_multi(u32* __restrict__ buf, const Mul
 // Big classes stage every group's signal in two rounds split by index bit
 // 9 or 11 (W * 8 KiB per group per round, 128 KiB in all).
 
-template <int T, int NT>
-__device__ __forceinline__ void multi_nat_read(u32 (&d)[NT][4], const u32* wl, u32 ntl, int lane) {
-    constexpr u32 L = 256u * T;
-    const int c16 = lane & 15, g = lane >> 4;
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const u32 m = (u32)(t / T) * L + (u32)(t % T) + T * (u32)c16 + 16u * T * (4u * g + j);
-            d[t][j] = (u32)t < ntl ? wl[nat_lds_word(m)] : 0u;
-        }
-}
-
 template <int T, int SIGN>
 __device__ __forceinline__ void multi_rows_nat(const u32* in, u32* out, const MultiTask& tk,
                                                const u32* __restrict__ offsets, const float2* w4096, u32* lds) {
@@ -1934,6 +1932,10 @@ __device__ __forceinline__ void multi_rows_nat(const u32* in, u32* out, const Mu
             const u32* src = in + offs[0] + 4u * lane;
 #pragma unroll
             for (int t = 0; t < 16; ++t) load4_nt(v[t], src + 256u * t);
+            // reader word of (t, j): base_j + (t ^ xlo_j) (nat_split_lane)
+            u32 base[4], xlo[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) nat_split_lane<8, 4>(16u * c16 + 256u * (4u * g + j), base[j], xlo[j]);
 #pragma unroll
             for (int r = 0; r < 2; ++r) {
 #pragma unroll
@@ -1943,8 +1945,7 @@ __device__ __forceinline__ void multi_rows_nat(const u32* in, u32* out, const Mu
 #pragma unroll
                 for (int t = 0; t < 16; ++t)
 #pragma unroll
-                    for (int j = r; j < 4; j += 2)
-                        d[t][j] = wl[nat_split_word<8>((u32)t + 16u * c16 + 256u * (4u * g + j))];
+                    for (int j = r; j < 4; j += 2) d[t][j] = wl[base[j] + ((u32)t ^ xlo[j])];
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             }
         };
@@ -1969,7 +1970,7 @@ __device__ __forceinline__ void multi_rows_nat(const u32* in, u32* out, const Mu
                         }
                     }
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-                multi_nat_read<T, NT>(d, wl, ntl, lane);
+                nat_rows_read<T, NT>(d, wl, ntl, lane);
                 __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
             };
             rows_body<T, false, SIGN, NT>(
@@ -2007,6 +2008,12 @@ __device__ __forceinline__ void multi_big_nat(const u32* in, u32* out, const Mul
     // 11 = j >> 1 for W = 4 and j & 1 for W = 8); for W = 16 bit 11 is
     // c16 >> 3, a lane predicate.
     constexpr int B = W == 2 ? 9 : 11;
+    constexpr int LW = W == 2 ? 5 : W == 4 ? 6 : W == 8 ? 7 : 8;  // log2(16W)
+    // reader word of (t, j): base_j + ((w + W t) ^ xlo_j) (nat_split_lane)
+    u32 base[4], xlo[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+        nat_split_lane<B, LW>(16u * W * (u32)c16 + 256u * W * (4u * g + (u32)j), base[j], xlo[j]);
     u32 d[16][4];
     u32* const region = lds + (u32)grp * (2048u * W);
 #pragma unroll
@@ -2021,11 +2028,11 @@ __device__ __forceinline__ void multi_big_nat(const u32* in, u32* out, const Mul
         for (int t = 0; t < 16; ++t)
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
-                const u32 n = (u32)w + W * ((u32)t + 16u * c16 + 256u * (4u * g + j));
+                const u32 word = base[j] + (((u32)w + W * (u32)t) ^ xlo[j]);
                 if constexpr (W == 16) {
-                    if ((c16 >> 3) == r) d[t][j] = region[nat_split_word<B>(n)];
+                    if ((c16 >> 3) == r) d[t][j] = region[word];
                 } else {
-                    if ((W == 4 ? (j >> 1) : (j & 1)) == r) d[t][j] = region[nat_split_word<B>(n)];
+                    if ((W == 4 ? (j >> 1) : (j & 1)) == r) d[t][j] = region[word];
                 }
             }
     }

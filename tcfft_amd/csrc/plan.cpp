@@ -457,12 +457,14 @@ tcfft::StartSched big_sched(const tcfft_plan_s* p, int W) {
 
 // The same for the natural-order single-pass kernel.  Its workgroups stage
 // the whole signal through LDS before any compute, so the 65536-point kernel
-// wants a longer stagger: 2^16 x 256 natural (3 groups) with delays 2, 4, 6
-// took 44.8, 42.8, 41.3 us on one box and 43.7, 42.5, 43.4 us on another;
-// for W = 2, 4, 8 the drop-in defaults stay within 2% of the best.
+// wants a longer stagger: with the start delay left at its default it runs 5
+// groups with a delay of 3 (2^16 x 256 natural, two runs: 3 groups with
+// delays 2, 3, 4, 5: 38.6-38.8, 38.2-38.4, 37.9-38.4, 37.1-37.4 us; 5 groups
+// with delay 2: 37.5-37.9, delay 3: 36.7-37.1; 7 groups with delay 2:
+// 36.8-37.0); for W = 2, 4, 8 the drop-in defaults stay within 2% of the best.
 tcfft::StartSched nat_big_sched(const tcfft_plan_s* p, int W) {
     if (p->big_delay >= 0 || W != 16) return big_sched(p, W);
-    return tcfft::StartSched{4u, (uint32_t)p->big_groups, 0u};
+    return tcfft::StartSched{3u, 5u, 0u};
 }
 
 // Start schedule of the contiguous-row first pass.  By default a delay of 1
