@@ -20,6 +20,8 @@ Prints one JSON line:
   cpu_fft_f32  = the single-precision host FFT the GPU path replaces, on this
                process's CPU share, bounded sample: FFTW3 API (libfftw3f,
                else MKL's wrappers) in a child process, else scipy.fft
+  natural_order = the same signals in natural order through tcfft_exec_natural
+               (out of place), its time, HBM fraction and rel-L2
   configs    = the other workloads (N=4096 x 1024; N=2^20 x 16; 2D 1024^2 x 8;
                split precision; uneven batch; FFT-conv layer), each with its
                time, GFLOP/s, HBM fraction and rel-L2 vs FP64, and hipFFT's
@@ -752,6 +754,19 @@ def main():
     dom = int(np.argmax(per_pass))
     alg_bytes = 8.0 * n * batch
     achieved = alg_bytes / (per_pass[dom] * 1e-3) / 1e9
+
+    # natural-order input (tcfft_exec_natural, out of place) on the same
+    # signals, under the natural kernel's own default start schedule
+    del bufs
+    plan.set_option(tcfft_amd.OPT_BIG_DELAY, -1)
+    nat = rotation(torch, (batch, n, 2), torch.float16, torch.from_numpy(np.ascontiguousarray(x)))
+    dst = torch.empty_like(nat[0])
+    plan.exec_natural(nat[0].data_ptr(), dst.data_ptr())
+    torch.cuda.synchronize()
+    nat_rel_l2 = rel(to_c(dst[:k]), ref64)
+    ms_nat = graph_time(torch, lambda i, st: plan.exec_natural(nat[i % len(nat)].data_ptr(), dst.data_ptr(),
+                                                               tcfft_amd.FORWARD, st), args.steps)
+    del nat, dst
     flops = 5.0 * n * math.log2(n) * batch
     out = {
         "metric": METRIC,
@@ -782,9 +797,11 @@ def main():
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": load_traffic(n, batch),
                      "kernel": f"pass {dom} of {plan.num_passes}"},
         "wall_s_timed": round(t_wall, 3),
+        "natural_order": {"ms": round(ms_nat, 5), "rel_l2_vs_fp64": nat_rel_l2,
+                          "hbm_frac": round(alg_bytes / (ms_nat * 1e-3) / 1e9 / PEAK_HBM_GBS, 4),
+                          "note": "tcfft_exec_natural, natural-order input, out of place, same signals"},
     }
     plan.destroy()
-    del bufs
     if not args.no_configs and n == N and batch == BATCH:
         reps = max(20, args.steps // 4)
         crng = np.random.default_rng(1)
