@@ -600,23 +600,25 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_cols(u32* __restrict_
 }
 
 // ----------------------------------------------------------------------------
-// k_cols_w: the strided radix-256 pass of k_cols<16> (no epilogue) for a
-// workgroup of 4 waves that owns 64 adjacent columns of one [256][s] block
-// (s >= 64), with 16-byte global accesses: the workgroup loads the 256 rows x
-// 256 bytes into LDS (16-byte lanes, four rows per wave instruction), each
-// wave reads its 16 columns in the B-form layout of k_cols<16> (lane (c16, g)
-// word i of tile j1 = row j0 + 16*j1, j0 = 4g+i), computes the same two
-// stages, writes its outputs back into the tile and the workgroup stores the
-// rows with 16-byte lanes.  LDS word of (row r, column c):
+// k_cols_w<T>: the strided radix-16T pass of k_cols<T> (T = 8, 16; no
+// epilogue) for a workgroup of 4 waves that owns 64 adjacent columns of one
+// [16T][s] block (s >= 64), with 16-byte global accesses: the workgroup loads
+// the 16T rows x 256 bytes into LDS (16-byte lanes, four rows per wave
+// instruction), each wave reads its 16 columns in the B-form layout of
+// k_cols<T> (lane (c16, g) word i of tile j1 = row j0 + 16*j1, j0 = 4g+i),
+// computes the same stages, writes its outputs back into the tile and the
+// workgroup stores the rows with 16-byte lanes.  LDS word of (row r, column c):
 // 64r + (c ^ 16*(((r >> 2) ^ (r >> 6)) & 3)): the 4-byte column reads (rows
 // 4g apart) and writes (rows 64g apart) spread the four lane groups over
 // different 16-bank groups, and a 16-byte piece of 4 columns stays
 // contiguous.
 __device__ __forceinline__ u32 colsw_word(u32 r, u32 c) { return 64u * r + (c ^ ((((r >> 2) ^ (r >> 6)) & 3u) << 4)); }
 
-template <int SIGN>
+template <int T, int SIGN>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_cols_w(u32* __restrict__ buf, u32 s_log2, u32 n_groups,
                                                                 const float2* __restrict__ w4096) {
+    static_assert(T == 8 || T == 16, "k_cols_w: radix 128 or 256");
+    constexpr u32 R = 16u * T;
     extern __shared__ u32 lds[];
     constexpr float sg = SIGN < 0 ? 1.f : -1.f;
     const u32 grp = blockIdx.x;
@@ -628,60 +630,92 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_cols_w(u32* __restric
     const u32 cg_log2 = s_log2 - 6u;  // 64-column groups per block
     const u32 blk = grp >> cg_log2;
     const u32 col0 = (grp & ((1u << cg_log2) - 1u)) * 64u;
-    u32* const base = buf + (((size_t)blk * 256u) << s_log2) + col0;
+    u32* const base = buf + (((size_t)blk * R) << s_log2) + col0;
     // load phase: thread (rr = tid >> 4, cq = tid & 15) takes rows rr + 16k,
     // columns 4cq..4cq+3
     const u32 rr = tid >> 4, cq = tid & 15u;
     float2 t_step[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) t_step[i] = w4096[16u * (4u * g + i)];  // W_256^{k0}
+    for (int i = 0; i < 4; ++i) t_step[i] = w4096[(256u / T) * (4u * g + i)];  // W_R^{k0}
     __builtin_amdgcn_sched_barrier(0);
     {
-        u32 v[16][4];
+        u32 v[T][4];
 #pragma unroll
-        for (int k = 0; k < 16; ++k) load4_nt(v[k], base + ((size_t)(rr + 16u * k) << s_log2) + 4u * cq);
+        for (int k = 0; k < T; ++k) load4_nt(v[k], base + ((size_t)(rr + 16u * k) << s_log2) + 4u * cq);
 #pragma unroll
-        for (int k = 0; k < 16; ++k) store4(lds + colsw_word(rr + 16u * k, 4u * cq), v[k]);
+        for (int k = 0; k < T; ++k) store4(lds + colsw_word(rr + 16u * k, 4u * cq), v[k]);
     }
     __syncthreads();
-    u32 d[16][4];
+    u32 d[T][4];
 #pragma unroll
-    for (int t = 0; t < 16; ++t)
+    for (int t = 0; t < T; ++t)
 #pragma unroll
         for (int i = 0; i < 4; ++i) d[t][i] = lds[colsw_word(16u * t + 4u * g + i, 16u * w + c16)];
     const Dft16Op op = make_dft16(lane, SIGN);
-    {
-        cf step[4], rec[4];
+    cf step[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) rec[i] = step[i] = cf{t_step[i].x, sg * t_step[i].y};
+    for (int i = 0; i < 4; ++i) step[i] = cf{t_step[i].x, sg * t_step[i].y};
+    if constexpr (T == 16) {
+        {
+            cf rec[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) rec[i] = step[i];
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                f32x4 re, im;
+                dft16_b(d[t], op, re, im);
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    cf v = {re[i], im[i]};
+                    if (t) {
+                        v = cmul(v, rec[i]);
+                        if (t < 15) rec[i] = cmul(rec[i], step[i]);
+                    }
+                    d[t][i] = pack(v);
+                }
+            }
+        }
+        u32 e[16][4];
+        transpose_gi_tile(d, e);  // (g,i) = j1, tile = k0
+        __syncthreads();          // every wave has read its columns
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
             f32x4 re, im;
+            dft16_b(e[t], op, re, im);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                lds[colsw_word((u32)t + 16u * (4u * g + i), 16u * w + c16)] = pack(cf{re[i], im[i]});
+        }
+    } else {
+        // stage 0 per tile (rows j0 = 4g + i, tile j1), then radix T across
+        // the tiles with W_R^{k0*j1}: output row k0 + 16*k1 (k0 = 4g + i)
+        cf acc[T][4];
+#pragma unroll
+        for (int t = 0; t < T; ++t) {
+            f32x4 re, im;
             dft16_b(d[t], op, re, im);
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                cf v = {re[i], im[i]};
-                if (t) {
-                    v = cmul(v, rec[i]);
-                    if (t < 15) rec[i] = cmul(rec[i], step[i]);
-                }
-                d[t][i] = pack(v);
-            }
+            for (int i = 0; i < 4; ++i) acc[t][i] = cf{re[i], im[i]};
         }
-    }
-    u32 e[16][4];
-    transpose_gi_tile(d, e);  // (g,i) = j1, tile = k0
-    __syncthreads();          // every wave has read its columns
+        __syncthreads();  // every wave has read its columns
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-        f32x4 re, im;
-        dft16_b(e[t], op, re, im);
+        for (int i = 0; i < 4; ++i) {
+            cf v[T];
+            v[0] = acc[0][i];
+            cf p = step[i];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) lds[colsw_word((u32)t + 16u * (4u * g + i), 16u * w + c16)] = pack(cf{re[i], im[i]});
+            for (int t = 1; t < T; ++t) {
+                v[t] = cmul(acc[t][i], p);
+                if (t + 1 < T) p = cmul(p, step[i]);
+            }
+            small_dft<T>(v, SIGN);
+#pragma unroll
+            for (int t = 0; t < T; ++t) lds[colsw_word(4u * g + i + 16u * t, 16u * w + c16)] = pack(v[t]);
+        }
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
+    for (int k = 0; k < T; ++k) {
         u32 v[4];
         load4(v, lds + colsw_word(rr + 16u * k, 4u * cq));
         store4_nt(base + ((size_t)(rr + 16u * k) << s_log2) + 4u * cq, v);
@@ -1554,11 +1588,11 @@ static hipError_t launch_cols_t(u32* buf, u32 s_log2, u32 outer, int sign, const
             return hipGetLastError();
         }
     }
-    if constexpr (T == 16) {
+    if constexpr (T == 8 || T == 16) {
         if (!epi && s_log2 >= 6) {
-            constexpr size_t lds = 256u * 64u * sizeof(u32);  // 256 rows x 64 columns
+            constexpr size_t lds = 16u * T * 64u * sizeof(u32);  // 16T rows x 64 columns
             const u32 groups = outer << (s_log2 - 6u);
-            auto k = sign < 0 ? k_cols_w<-1> : k_cols_w<+1>;
+            auto k = sign < 0 ? k_cols_w<T, -1> : k_cols_w<T, +1>;
             const KernelInfo ki = kernel_info(k, 256, lds);
             if (ki.lds_status != hipSuccess) return ki.lds_status;
             hipLaunchKernelGGL(k, dim3(groups), dim3(256), lds, st, buf, s_log2, groups, w4096);

@@ -107,7 +107,7 @@ def test_multipass_forward_vs_oracle(n, batch):
 
 @pytest.mark.parametrize("n,batch,direction", [(1 << 19, 1, tcfft_amd.FORWARD), (1 << 20, 2, tcfft_amd.FORWARD),
                                                (1 << 20, 1, tcfft_amd.INVERSE), (1 << 21, 1, tcfft_amd.FORWARD),
-                                               (1 << 22, 1, tcfft_amd.FORWARD),
+                                               (1 << 22, 1, tcfft_amd.FORWARD), (1 << 23, 1, tcfft_amd.INVERSE),
                                                (1 << 24, 1, tcfft_amd.FORWARD), (1 << 25, 1, tcfft_amd.INVERSE),
                                                (1 << 26, 1, tcfft_amd.FORWARD), (1 << 27, 1, tcfft_amd.INVERSE)])
 def test_large_vs_numpy(n, batch, direction):
@@ -195,10 +195,11 @@ def test_golden_vectors(golden_dir, n):
         assert oracle.rel_l2(run_1d(x, tcfft_amd.INVERSE), d["adjoint"]) <= TOL
 
 
-# nx = 512, 1024, 2048: the column axis in one pass of radix 256*W
+# nx = 512, 1024, 2048: the column axis in one pass of radix 256*W; nx = 128,
+# 256: the 64-column LDS-tile pass of radix 128 / 256
 @pytest.mark.parametrize("shape,batch", [((256, 256), 2), ((512, 1024), 1), ((1024, 1024), 1), ((16, 256), 3),
                                          ((2, 512), 2), ((256, 8192), 1), ((2048, 256), 2), ((512, 4096), 1),
-                                         ((4096, 256), 1)])
+                                         ((4096, 256), 1), ((128, 256), 3), ((128, 4096), 1)])
 def test_2d_vs_oracle(shape, batch):
     nx, ny = shape
     x = fp16_input((batch, nx, ny), seed=nx * 7 + ny)
