@@ -219,7 +219,8 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * tcfft_exec_split (0 = whole batch, the default), which bounds the
  * intermediate's footprint (smaller chunks keep it in the last-level cache
  * but leave compute units idle; at 2^16 x 256 the whole batch is fastest).
- * TCFFT_OPT_BIG_DELAY (-1..64, default -1 = per-kernel default) and
+ * TCFFT_OPT_BIG_DELAY (-1..64, default -1 = per-kernel default; the
+ * natural-order 65536-point kernel then runs 5 groups with delay 3) and
  * TCFFT_OPT_BIG_GROUPS (1..8, default 3): start schedule of the first pass
  * (contiguous lines of up to 65536 points); the workgroups resident at
  * launch form GROUPS groups and group k starts k * DELAY quanta (1024 GPU
@@ -233,8 +234,8 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * over at least 32 waves per compute unit, else 1): waves per workgroup of a
  * contiguous-row first pass (lines of 256..4096 points, one wave per 16
  * tiles of 256 points), i.e. how the waves are grouped onto compute units.
- * TCFFT_OPT_FIRST_CACHED (-1, the default: cached while the transform's
- * buffer is at most 256 MiB; 0 = never; 1 = always): whether the first pass
+ * TCFFT_OPT_FIRST_CACHED (-1, the default: cached while the signals of one
+ * batch chunk take at most 256 MiB; 0 = never; 1 = always): whether the first pass
  * of a multi-pass plan stores its output through the caches, so that the
  * next pass finds much of it in the 256 MiB last-level cache (MALL), instead
  * of with non-temporal stores.
@@ -249,7 +250,7 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * points without a folded twiddle (at least the line's tiles; 4096-point
  * lines always take 16).  The default is the largest that still gives every
  * SIMD two waves (16 on all but small batches).
- * The schedule never changes results. */
+ * None of these options changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,
     TCFFT_OPT_BIG_DELAY = 2,
