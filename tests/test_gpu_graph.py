@@ -1,6 +1,6 @@
 """hipGraph capture of the entry points that allocate nothing: the drop-in
 exec (1D and 2D), natural order (out of place, and in place through the
-plan's workspace), the uneven batch, split
+plan's workspace), transforms run in batch chunks, the uneven batch, split
 precision, the real-input transforms and the fused convolution.  Each graph is replayed three times with
 eager work (input restore, an eager transform of another buffer) between the
 replays, and every replay must equal the eager result bit for bit."""
@@ -112,6 +112,42 @@ def test_graph_natural_in_place(shape):
     replay_check(lambda st: plan.exec_natural(b.data_ptr(), b.data_ptr(), tcfft_amd.FORWARD, st),
                  lambda: b.copy_(x), b, want,
                  lambda: plan.exec_natural(scratch.copy_(x).data_ptr(), scratch.data_ptr()))
+    plan.destroy()
+
+
+@pytest.mark.parametrize("shape", [(5, 1 << 20), (2, 65536, 256)], ids=["1d-2^20x5", "2d-65536x256x2"])
+def test_graph_batch_chunks(shape):
+    """A transform run in batch chunks (TCFFT_OPT_MALL_CHUNK at 16 MiB: 3 + 2
+    signals of 2^20, one 2D image of 64 MiB per chunk at 64 MiB) captures as
+    one graph of all chunks' launches, drop-in and natural order; every
+    replay equals the eager whole-batch result."""
+    note(f"[gpu] graph: batch chunks {shape}")
+    rng = np.random.default_rng(sum(shape))
+    if len(shape) == 2:
+        plan, mib = tcfft_amd.Plan1D(shape[1], shape[0]), 16
+    else:
+        plan, mib = tcfft_amd.Plan2D(shape[1], shape[2], shape[0]), 64
+    x = fp16(rng, shape + (2,))
+    plan.set_option(tcfft_amd.OPT_MALL_CHUNK, 0)
+    want = x.clone()
+    plan.exec(want.data_ptr())
+    want_nat = torch.empty_like(x)
+    plan.exec_natural(x.data_ptr(), want_nat.data_ptr())
+    torch.cuda.synchronize()
+    plan.set_option(tcfft_amd.OPT_MALL_CHUNK, mib)
+    b, nat = x.clone(), torch.empty_like(x)
+    scratch = torch.empty_like(x)
+
+    def reset():
+        b.copy_(x)
+        nat.zero_()
+
+    def launch(st):
+        plan.exec(b.data_ptr(), tcfft_amd.FORWARD, st)
+        plan.exec_natural(x.data_ptr(), nat.data_ptr(), tcfft_amd.FORWARD, st)
+
+    replay_check(launch, reset, b, want, lambda: plan.exec(scratch.copy_(x).data_ptr()))
+    assert torch.equal(nat, want_nat)
     plan.destroy()
 
 
