@@ -7,6 +7,9 @@ fp16 values, in place, digit-reversed input, natural output) on one MI355X.
 Inputs are synthetic uniform[-1,1) fp16 and already resident in HBM; the timed
 steps rotate over buffers totalling > 640 MiB so no step starts with its data
 in the 256 MB MALL.  Steps are captured once into a hipGraph and replayed.
+Every in-place workload transforms each buffer at most once per timed region
+and gets its input back before the next (Rotation.refresh): repeated in place,
+fp16 data overflows to inf/NaN, which runs faster than finite data.
 
 Prints one JSON line:
   value      = 5*N*log2(N)*batch / step time, GFLOP/s
@@ -76,22 +79,27 @@ def parse():
 SPIN_CYCLES = 250_000
 
 
-def graph_time(torch, launch, reps, replays=1):
+def graph_time(torch, launch, reps, replays=1, refresh=None):
     """ms per launch(i, stream): reps launches captured once into a hipGraph,
     replayed `replays` times between synchronizations (replay() runs on the
     current stream, so the events are recorded there); median over replays.
     Each replay is queued behind a short spin kernel (SPIN_CYCLES) that ends
-    before the start event."""
+    before the start event.  refresh(), if given, runs before every replay,
+    outside the timed region (in-place workloads put their input back)."""
     g = torch.cuda.CUDAGraph()
     cs = torch.cuda.Stream()
     cs.wait_stream(torch.cuda.current_stream())
     with torch.cuda.graph(g, stream=cs):
         for i in range(reps):
             launch(i, cs.cuda_stream)
+    if refresh:
+        refresh()
     g.replay()
     torch.cuda.synchronize()
     times = []
     for _ in range(replays):
+        if refresh:
+            refresh()
         a = torch.cuda.Event(enable_timing=True)
         b = torch.cuda.Event(enable_timing=True)
         torch.cuda._sleep(SPIN_CYCLES)
@@ -103,14 +111,16 @@ def graph_time(torch, launch, reps, replays=1):
     return float(np.median(times))
 
 
-def eager_time(torch, fn, reps, spin=SPIN_CYCLES):
+def eager_time(torch, fn, reps, spin=SPIN_CYCLES, refresh=None):
     """ms per fn(i) over reps eager calls on the current stream.  As in
     graph_time, a spin of `spin` GPU cycles is queued ahead of the start
     event, so that the region holds the calls' GPU work back to back and not
     the host's latency before the first launch (callers whose Python-side
-    call is slow pass a longer spin)."""
+    call is slow pass a longer spin); refresh() runs first, if given."""
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
+    if refresh:
+        refresh()
     torch.cuda.synchronize()
     torch.cuda._sleep(spin)
     a.record()
@@ -121,14 +131,45 @@ def eager_time(torch, fn, reps, spin=SPIN_CYCLES):
     return a.elapsed_time(b) / reps
 
 
-def rotation(torch, shape, dtype, init):
-    """Buffers of `shape` whose total exceeds ROTATE_BYTES, each = init."""
-    one = int(np.prod(shape)) * torch.tensor([], dtype=dtype).element_size()
-    k = max(2, -(-ROTATE_BYTES // one))
-    bufs = [torch.empty(shape, dtype=dtype, device="cuda") for _ in range(k)]
-    for b in bufs:
-        b.copy_(init)
-    return bufs
+_READ_FLUSH = []
+
+
+def read_flush(torch):
+    """Reads 512 MiB of a buffer of its own (a reduction), so that what a
+    refresh has just written leaves the 256 MB MALL, written back, before
+    the timed region instead of inside it."""
+    if not _READ_FLUSH:
+        _READ_FLUSH.append(torch.ones(128 << 20, dtype=torch.float32, device="cuda"))
+    _READ_FLUSH[0].sum()
+
+
+class Rotation(list):
+    """Buffers of one shape whose total exceeds ROTATE_BYTES, and at least
+    `count` of them, each = init.  Transforms run in place; repeated on one
+    buffer they grow fp16 values to inf/NaN within two or three transforms,
+    and such data runs faster than finite data (a few percent on this part).
+    refresh() puts init back into every buffer (from a device copy), then
+    runs read_flush: a timed region with at most one transform per buffer
+    then sees finite data, cold in the MALL."""
+
+    def __init__(self, torch, shape, dtype, init, count=0):
+        one = int(np.prod(shape)) * torch.tensor([], dtype=dtype).element_size()
+        k = max(2, count, -(-ROTATE_BYTES // one))
+        self.torch = torch
+        self.src = init.to(device="cuda", dtype=dtype).reshape(shape)
+        super().__init__(torch.empty(shape, dtype=dtype, device="cuda") for _ in range(k))
+        for b in self:
+            b.copy_(self.src)
+
+    def refresh(self):
+        for b in self:
+            b.copy_(self.src)
+        read_flush(self.torch)
+
+
+def rotation(torch, shape, dtype, init, count=0):
+    """Rotation(torch, shape, dtype, init, count)."""
+    return Rotation(torch, shape, dtype, init, count)
 
 
 def rel(a, b):
@@ -307,14 +348,15 @@ def bench_plan1d(torch, tcfft_amd, n, batch, reps, rng):
     plan = tcfft_amd.Plan1D(n, batch)
     x = fp16_uniform(rng, (batch, n))
     xr = np.ascontiguousarray(x[:, tcfft_amd.gen_rev(n, plan.radices)])
-    bufs = rotation(torch, (batch, n, 2), torch.float16, torch.from_numpy(xr))
+    bufs = rotation(torch, (batch, n, 2), torch.float16, torch.from_numpy(xr), count=reps)
     sched = plan.autotune(bufs[0].data_ptr())[:2]  # start schedule on this box (cold data)
     bufs[0].copy_(torch.from_numpy(xr))
     plan.exec(bufs[0].data_ptr())
     torch.cuda.synchronize()
     k = min(batch, 4)
     err = rel(to_c(bufs[0][:k]), np.fft.fft(x[:k, :, 0].astype(np.float64) + 1j * x[:k, :, 1], axis=-1))
-    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3)
+    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3,
+                    refresh=bufs.refresh)
     out = summary(ms, n, batch, 8.0 * n * batch)
     out.update({"rel_l2_vs_fp64": err, "passes": plan.num_passes, "radices": plan.radices,
                 "start_schedule": list(sched)})
@@ -335,13 +377,14 @@ def bench_plan2d(torch, tcfft_amd, nx, ny, batch, reps, rng):
     rx = tcfft_amd.gen_rev(nx, plan.radices_x)
     ry = tcfft_amd.gen_rev(ny, plan.radices_y)
     xr = np.ascontiguousarray(x[:, rx][:, :, ry])
-    bufs = rotation(torch, (batch, nx, ny, 2), torch.float16, torch.from_numpy(xr))
+    bufs = rotation(torch, (batch, nx, ny, 2), torch.float16, torch.from_numpy(xr), count=reps)
     sched = plan.autotune(bufs[0].data_ptr())[:2]  # start schedules on this box (cold data)
     bufs[0].copy_(torch.from_numpy(xr))
     plan.exec(bufs[0].data_ptr())
     torch.cuda.synchronize()
     err = rel(to_c(bufs[0][:1]), np.fft.fft2(x[:1, :, :, 0].astype(np.float64) + 1j * x[:1, :, :, 1]))
-    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3)
+    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3,
+                    refresh=bufs.refresh)
     out = summary(ms, nx * ny, batch, 8.0 * nx * ny * batch)
     out.update({"rel_l2_vs_fp64": err, "passes": plan.num_passes, "radices_x": plan.radices_x,
                 "radices_y": plan.radices_y, "start_schedule": list(sched)})
@@ -377,8 +420,10 @@ def bench_split(torch, tcfft_amd, reps, chunk=None, ms_plain=None):
     ms = graph_time(torch, lambda i, st: plan.exec_split(ins[i % len(ins)].data_ptr(), outs[i % len(ins)].data_ptr(),
                                                          tcfft_amd.FORWARD, st), reps, 3)
     if ms_plain is None:
-        pb = rotation(torch, (batch, n, 2), torch.float16, torch.zeros(batch, n, 2, dtype=torch.float16))
-        ms_plain = graph_time(torch, lambda i, st: plan.exec(pb[i % len(pb)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3)
+        pb = rotation(torch, (batch, n, 2), torch.float16,
+                      torch.from_numpy(fp16_uniform(np.random.default_rng(1), (batch, n))), count=reps)
+        ms_plain = graph_time(torch, lambda i, st: plan.exec(pb[i % len(pb)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3,
+                              refresh=pb.refresh)
     plan.destroy()
     out = summary(ms, n, batch, 16.0 * n * batch)
     out.update({"rel_l2_max_per_signal": max(errs), "ms_plain_fp16": round(ms_plain, 5),
@@ -429,12 +474,11 @@ def bench_uneven(torch, tcfft_amd, reps):
     flat = rng.uniform(-1, 1, (total, 2)).astype(np.float16)
     mp = tcfft_amd.MultiPlan(sizes)
     host = torch.from_numpy(flat)
-    bufs = rotation(torch, tuple(host.shape), torch.float16, host)
+    bufs = rotation(torch, tuple(host.shape), torch.float16, host, count=reps)
     g, o, _ = mp.autotune(bufs[0].data_ptr())
     delay = mp.schedule()[2]
-    for b in bufs:
-        b.copy_(host)
-    ms_multi = graph_time(torch, lambda i, st: mp.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3)
+    ms_multi = graph_time(torch, lambda i, st: mp.exec(bufs[i % len(bufs)].data_ptr(), tcfft_amd.FORWARD, st), reps, 3,
+                          refresh=bufs.refresh)
     classes = sorted(set(int(v) for v in sizes))
     plans, offs, o2 = [], [], 0
     for n in classes:
@@ -448,11 +492,11 @@ def bench_uneven(torch, tcfft_amd, reps):
         for p, off in zip(plans, offs):
             p.exec(base + 4 * off, tcfft_amd.FORWARD, st)
 
-    ms_sizes = graph_time(torch, per_size, reps, 3)
+    ms_sizes = graph_time(torch, per_size, reps, 3, refresh=bufs.refresh)
     # natural-order signals (tcfft_multi_exec_natural, out of place, the same schedule)
     nat_out = torch.empty_like(bufs[0])
     ms_nat = graph_time(torch, lambda i, st: mp.exec_natural(bufs[i % len(bufs)].data_ptr(), nat_out.data_ptr(),
-                                                               tcfft_amd.FORWARD, st), reps, 3)
+                                                               tcfft_amd.FORWARD, st), reps, 3, refresh=bufs.refresh)
     # accuracy on a sample of signals, digit-reversed per signal
     sample = [int(i) for i in rng.choice(n_sig, 16, replace=False)]
     check = torch.from_numpy(flat).cuda()
@@ -607,7 +651,7 @@ def hipfft_fp16(torch, dims, batch, reps, fp32=False):
     x = fp16_uniform(rng, (batch,) + tuple(dims))
     if fp32:
         x = x.astype(np.float32)
-    bufs = rotation(torch, x.shape, torch.float32 if fp32 else torch.float16, torch.from_numpy(x))
+    bufs = rotation(torch, x.shape, torch.float32 if fp32 else torch.float16, torch.from_numpy(x), count=reps)
     lib.hipfftXtExec(h, ctypes.c_void_p(bufs[0].data_ptr()), ctypes.c_void_p(bufs[0].data_ptr()), -1)
     torch.cuda.synchronize()
     k = min(batch, 2)
@@ -616,7 +660,8 @@ def hipfft_fp16(torch, dims, batch, reps, fp32=False):
     err = rel(to_c(bufs[0][:k]), ref)
     nb = len(bufs)
     ms = eager_time(torch, lambda i: lib.hipfftXtExec(h, ctypes.c_void_p(bufs[i % nb].data_ptr()),
-                                                       ctypes.c_void_p(bufs[i % nb].data_ptr()), -1), reps)
+                                                       ctypes.c_void_p(bufs[i % nb].data_ptr()), -1), reps,
+                    refresh=bufs.refresh)
     lib.hipfftDestroy(h)
     out = summary(ms, total, batch, (16.0 if fp32 else 8.0) * total * batch)
     out.update({"rel_l2_vs_fp64": err,
@@ -714,7 +759,8 @@ def main():
     x = fp16_uniform(rng, (batch, n))
     x_rev = np.ascontiguousarray(x[:, tcfft_amd.gen_rev(n, plan.radices)])
     host = torch.from_numpy(x_rev)
-    bufs = rotation(torch, (batch, n, 2), torch.float16, host)
+    # one transform per buffer per timed region (per-pass timing: 100 launches)
+    bufs = rotation(torch, (batch, n, 2), torch.float16, host, count=max(args.steps, 100))
     torch.cuda.synchronize()
     if args.calib:
         for _ in range(5):
@@ -741,8 +787,9 @@ def main():
         plan.exec(bufs[i % nb].data_ptr())
     torch.cuda.synchronize()
     t_wall0 = time.perf_counter()
-    ms_eager = eager_time(torch, lambda i: plan.exec(bufs[i % nb].data_ptr()), args.steps)
-    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % nb].data_ptr(), tcfft_amd.FORWARD, st), args.steps)
+    ms_eager = eager_time(torch, lambda i: plan.exec(bufs[i % nb].data_ptr()), args.steps, refresh=bufs.refresh)
+    ms = graph_time(torch, lambda i, st: plan.exec(bufs[i % nb].data_ptr(), tcfft_amd.FORWARD, st), args.steps,
+                    refresh=bufs.refresh)
     t_wall = time.perf_counter() - t_wall0
 
     # per-pass timing (each pass kernel alone, graph-replayed on its stream)
@@ -750,7 +797,7 @@ def main():
     for p in range(plan.num_passes):
         per_pass.append(graph_time(torch, lambda i, st, p=p: plan.exec_passes(bufs[i % nb].data_ptr(), p, 1,
                                                                               tcfft_amd.FORWARD, st),
-                                   max(100, args.steps), 5))
+                                   max(100, args.steps), 5, refresh=bufs.refresh))
     dom = int(np.argmax(per_pass))
     alg_bytes = 8.0 * n * batch
     achieved = alg_bytes / (per_pass[dom] * 1e-3) / 1e9
@@ -782,7 +829,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f16 (FP32 MFMA accumulate)",
-        "data": f"synthetic uniform[-1,1) fp16, resident in HBM, rotated over {nb} buffers",
+        "data": f"synthetic uniform[-1,1) fp16, resident in HBM, rotated over {nb} buffers, each transformed "
+                "at most once per timed region (input restored before each)",
         "config": {"workload": f"1D C2C FFT N={n} batch={batch}, fp16 interleaved, in place, "
                                f"digit-reversed input (tcfftExec)", "n": n, "batch": batch,
                    "passes": plan.num_passes, "radices": plan.radices,

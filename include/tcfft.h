@@ -246,10 +246,11 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * radix-512..2048 column pass (2D nx = 512..2048, 1D 2^25..2^27) and batches
  * whose single signal exceeds the size are not chunked.
  * TCFFT_OPT_ROWS_TILES (4, 8 or 16; 0, the default, = by the grid): tiles of
- * 256 points per wave of a contiguous-row first pass of lines up to 2048
- * points without a folded twiddle (at least the line's tiles; 4096-point
- * lines always take 16).  The default is the largest that still gives every
- * SIMD two waves (16 on all but small batches).
+ * 256 points per wave of a contiguous-row first pass without a folded
+ * twiddle (at least the line's tiles, and 8 or 16 for 4096-point lines, 8
+ * meaning two waves per line).  The default is the largest that still gives
+ * every SIMD two waves (16 on all but small batches; 16 for 4096-point
+ * lines).
  * None of these options changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,
@@ -266,17 +267,19 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
 
 /* Times the plan's forward transform under each start schedule candidate
  * of its first pass (no stagger, or 3, 5 or 7 groups with delays of 1..4
- * quanta) on `data` (which it overwrites with transforms of itself), keeps
- * the fastest in the plan and reports it; then, with that schedule, the
- * workgroup size of a contiguous-row first pass (TCFFT_OPT_ROWS_WAVES), its
- * tiles per wave (TCFFT_OPT_ROWS_TILES, lines up to 2048 points) and the
+ * quanta) on `data`, keeps the fastest in the plan and reports it; then,
+ * with that schedule, the workgroup size of a contiguous-row first pass
+ * (TCFFT_OPT_ROWS_WAVES), its tiles per wave (TCFFT_OPT_ROWS_TILES) and the
  * column pass delay (0..4) when the plan has a strided pass of radix
- * 512..2048.  Every timed run
- * follows a write of a 256 MiB buffer that the call allocates and frees
- * (skipped if the allocation fails), so that the data is timed cold, as a
- * caller's previous work leaves it; the four fastest candidates are timed
- * again in three alternating rounds before the choice.  No reference
- * counterpart (the reference has one fixed launch per size). */
+ * 512..2048.  Every timed run starts from the caller's data (a copy that
+ * the call allocates: transforms repeated in place would time inf/NaN
+ * values, which run faster than finite data) and follows a write of a
+ * 256 MiB buffer that the call allocates and frees, so that the data is
+ * timed cold, as a caller's previous work leaves it; the four fastest
+ * candidates are timed again in three alternating rounds before the choice.
+ * On return `data` holds what it held on entry (if the copy cannot be
+ * allocated, transforms of itself, and the timed runs see those).  No
+ * reference counterpart (the reference has one fixed launch per size). */
 int tcfft_plan_autotune(tcfft_plan plan, void *data, void *stream, int *best_groups, int *best_delay,
                         float *best_ms);
 
@@ -309,11 +312,11 @@ int tcfft_multi_set_schedule(tcfft_multi plan, int grid, int order);
  * Never changes results. */
 int tcfft_multi_set_start_delay(tcfft_multi plan, int delay);
 int tcfft_multi_get_schedule(tcfft_multi plan, int *grid, int *order, int *delay);
-/* Times every (grid, order, start delay 0..2) candidate on `data` (which it
- * overwrites), times the four fastest again in three alternating rounds,
- * and keeps the one with the lowest time.  Each timed run follows a write of
- * a 256 MiB buffer that the call allocates and frees (skipped if the
- * allocation fails), so that the data is timed cold, not cache-resident. */
+/* Times every (grid, order, start delay 0..2) candidate on `data`, times the
+ * four fastest again in three alternating rounds, and keeps the one with the
+ * lowest time.  As in tcfft_plan_autotune, each timed run starts from a copy
+ * of the caller's data and follows a write of a 256 MiB buffer (each
+ * skipped if its allocation fails), and `data` is restored on return. */
 int tcfft_multi_autotune(tcfft_multi plan, void *data, void *stream, int *best_grid, int *best_order,
                          float *best_ms);
 

@@ -197,6 +197,112 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows(u32* __restrict_
 }
 
 // ----------------------------------------------------------------------------
+// k_rows_half: 4096-point lines (no epilogue) with two waves per line, eight
+// tiles each, so that a small batch puts twice the waves, and each wave half
+// the load latency, on the chip.  Wave h of a line loads tiles d2 = 8h..8h+7
+// in k_rows<16>'s layout and runs stages 0 and 1 of dit4096 on them, with
+// the stage-1 recurrence W_4096^{c*d2} advanced over tiles 1..8h-1 first, so
+// that every product equals the one-wave kernel's (bit-identical results).
+// The stage-2 inputs go through LDS, [d2][lds_sigma(c)] per line (16 KiB):
+// written as 16-byte rows of four columns c = 4g + i + 16*c16, read back by
+// wave h as columns c = j + 4*c16 + 64q + 128h (q = 0, 1) of the 16 points
+// d2 = 4g + i, the operand of the B-form radix 16 over d2; output
+// X[c + 256*k2], k2 = 4g + i, leaves as 16-byte stores of four columns.
+// lines_per_wg lines per workgroup (2 waves each); a line beyond n_lines
+// loads the last line, keeps the barrier and stores nothing.
+template <int SIGN, bool CACHED>
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half(u32* __restrict__ buf, u32 n_lines,
+                                                                   const float2* __restrict__ w4096,
+                                                                   StartSched sched) {
+    extern __shared__ u32 lds[];
+    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+    const u32 wv = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 h = wv & 1u, ln = wv >> 1;
+    const u32 line = blockIdx.x * (blockDim.x >> 7) + ln;
+    const bool active = line < n_lines;
+    start_delay(sched);
+    u32* const lbase = buf + (size_t)(active ? line : n_lines - 1u) * 4096u;
+    u32* const xl = lds + ln * 4096u;
+
+    Dit4096Tw tw;
+    dit4096_tw_load(tw, lane, w4096);
+    __builtin_amdgcn_sched_barrier(0);
+    u32 d[8][4];
+    u32* const tin = lbase + 2048u * h + 16u * c16 + 4u * g;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        load4_nt(d[t], tin + 256 * t);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    const Dft16Op op = make_dft16(lane, SIGN);
+    cf tw1[4], step[4], rec[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tw1[i] = cf{tw.tw1[i].x, sg * tw.tw1[i].y};
+        step[i] = cf{tw.step[i].x, sg * tw.step[i].y};
+        rec[i] = step[i];
+    }
+    if (h) {
+        // rec = W_4096^{8c}, by the one-wave kernel's chain of products
+#pragma unroll
+        for (int k = 1; k < 8; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) rec[i] = cmul(rec[i], step[i]);
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        f32x4 re, im;
+        dft16_a(d[t], op, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
+        dft16_a(d[t], op, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            cf v = {re[i], im[i]};
+            if (h || t) {
+                v = cmul(v, rec[i]);
+                if (t < 7) rec[i] = cmul(rec[i], step[i]);
+            }
+            d[t][i] = pack(v);
+        }
+        // tile d2 = 8h + t: lane (k1 = c16, k0 = 4g + i), columns 4g + 16*c16 + i
+        store4(xl + (8u * h + (u32)t) * 256u + lds_sigma(16u * c16 + 4u * g), d[t]);
+    }
+    __syncthreads();
+    u32 v[2][4][4];  // [q][i: point d2 = 4g + i][j: column]
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) load4(v[q][i], xl + (4u * g + i) * 256u + lds_sigma(4u * c16 + 64u * q + 128u * h));
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        u32 o[4][4];  // [j][i: k2 = 4g + i]
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            u32 a[4] = {v[q][0][j], v[q][1][j], v[q][2][j], v[q][3][j]};
+            f32x4 re, im;
+            dft16_b(a, op, re, im);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o[j][i] = pack(cf{re[i], im[i]});
+        }
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                u32 s[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
+                u32* const p = lbase + 256u * (4u * g + i) + 4u * c16 + 64u * q + 128u * h;
+                if constexpr (CACHED)
+                    store4(p, s);
+                else
+                    store4_nt(p, s);
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------
 // k_rows_nat<T>: the first pass of k_rows<T> on natural-order input (the
 // digit reversal of the caller's gen_rev done in the load), in -> out, in
 // place allowed.  Each wave loads its 16 tiles (whole lines) with 16-byte
@@ -1538,14 +1644,38 @@ static hipError_t launch_rows_t(u32* buf, u32 n_tiles, const float2* w4096, cons
     return hipGetLastError();
 }
 
+// k_rows_half over n_tiles / 16 lines: wg = 4 puts two lines in a workgroup,
+// 1 or 2 one line.
+template <int SIGN, bool CACHED>
+static hipError_t launch_rows_half_t(u32* buf, u32 n_tiles, const float2* w4096, StartSched sched, u32 wg,
+                                     hipStream_t st) {
+    const u32 lines = n_tiles / 16u, lpw = wg == 4u ? 2u : 1u;
+    const size_t lds = (size_t)lpw * 4096u * sizeof(u32);
+    auto k = k_rows_half<SIGN, CACHED>;
+    const KernelInfo ki = kernel_info(k, (int)(128 * lpw), lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    if (sched.groups == 0) sched.delay = 0;
+    if (sched.delay) sched.resident = ki.resident;
+    hipLaunchKernelGGL(k, dim3((lines + lpw - 1u) / lpw), dim3(128 * lpw), lds, st, buf, lines, w4096, sched);
+    return hipGetLastError();
+}
+
 // Only the 4096-point lines (T = 16) have non-temporal output stores to
 // switch; shorter lines always store through the caches.  Tiles per wave:
 // 16, or 8 / 4 (at least T) for lines of up to 2048 points without an
-// epilogue, which puts more waves on a small grid.
+// epilogue, which puts more waves on a small grid; for 4096-point lines 8
+// tiles per wave is k_rows_half, two waves per line.
 template <int T>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
                                 StartSched sched, u32 wg, u32 tiles, hipStream_t st) {
     if constexpr (T == 16) {
+        if (tiles == 8) {
+            if (epi) return hipErrorInvalidValue;  // the epilogue pass runs 16 tiles per wave only
+            return sign < 0 ? (sched.cached_out ? launch_rows_half_t<-1, true>(buf, n_tiles, w4096, sched, wg, st)
+                                                : launch_rows_half_t<-1, false>(buf, n_tiles, w4096, sched, wg, st))
+                            : (sched.cached_out ? launch_rows_half_t<+1, true>(buf, n_tiles, w4096, sched, wg, st)
+                                                : launch_rows_half_t<+1, false>(buf, n_tiles, w4096, sched, wg, st));
+        }
         if (tiles != 16) return hipErrorInvalidValue;
         if (sched.cached_out)
             return sign < 0 ? launch_rows_t<T, -1, true>(buf, n_tiles, w4096, epi, sched, wg, st)

@@ -485,7 +485,12 @@ uint32_t rows_wg(const tcfft_plan_s* p, const Pass& ps) {
 }
 
 // Tiles (of 256 points) per wave of a contiguous-row first pass: 16 for
-// 4096-point lines and passes with an epilogue; otherwise the plan's setting,
+// passes with an epilogue; for 4096-point lines 16, or 8 (two waves per
+// line, k_rows_half) when the plan sets 4 or 8, or by default when one wave
+// per line gives at most one wave per SIMD (random input, rotated buffers:
+// 4096 x 1024 8.3-8.4 -> 7.8 us, 4096 x 256 5.7 -> 4.9 us; at 2048 lines
+// and more one wave per line is ahead, 12.1 against 14.6 us, and so is 2D
+// 256 x 4096 x 8 with 2048 lines, 27.9 against 28.8); otherwise the plan's setting,
 // or by default the largest of 16, 8, 4 (at least the line's tiles) that
 // still gives every SIMD two waves, so that a small grid spreads over the
 // chip.  Measured (random input; 16 / 8 / 4 tiles): 1024 x 1024 6.17 / 5.23
@@ -495,7 +500,11 @@ uint32_t rows_wg(const tcfft_plan_s* p, const Pass& ps) {
 // against 16.7 and 18.3).
 uint32_t rows_nt(const tcfft_plan_s* p, const Pass& ps) {
     const uint32_t T = (uint32_t)ps.radix;
-    if (T >= 16 || ps.has_epi) return 16u;
+    if (ps.has_epi) return 16u;
+    if (T >= 16) {
+        if (p->rows_tiles > 0) return p->rows_tiles == 16 ? 16u : 8u;
+        return ps.n_tiles / 16u <= 4u * (uint32_t)p->cus ? 8u : 16u;
+    }
     const uint32_t lo = T <= 4 ? 4u : 8u;
     if (p->rows_tiles > 0) return std::max((uint32_t)p->rows_tiles, lo);
     const uint32_t want = 8u * (uint32_t)p->cus;  // two waves per SIMD
@@ -504,8 +513,9 @@ uint32_t rows_nt(const tcfft_plan_s* p, const Pass& ps) {
     return lo;
 }
 
-tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps) {
-    const uint32_t nt = rows_nt(p, ps);
+// Start schedule of a contiguous-row first pass run with nt tiles per wave
+// (the natural-order kernel k_rows_nat always runs 16).
+tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps, uint32_t nt) {
     const uint32_t waves = (ps.n_tiles + nt - 1u) / nt;  // at most one wave per SIMD: a delay of 1
     const int delay = p->big_delay >= 0 ? p->big_delay : (waves <= 4u * (uint32_t)p->cus ? 1 : 0);
     return tcfft::StartSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
@@ -564,7 +574,7 @@ int run_passes(tcfft_plan_s* p, uint32_t* buf, int sign, hipStream_t st, size_t 
         const EpiParams* epi = ps.has_epi ? &ps.epi : nullptr;
         hipError_t e;
         if (ps.kind == 0) {
-            tcfft::StartSched sc = rows_sched(p, ps);
+            tcfft::StartSched sc = rows_sched(p, ps, rows_nt(p, ps));
             sc.cached_out = cached_out(p, i, nsig);
             e = tcfft::launch_rows(ps.radix, buf, ps.n_tiles, sign, p->w4096, epi, sc, rows_wg(p, ps), rows_nt(p, ps),
                                    st);
@@ -1080,6 +1090,42 @@ int dif_params(tcfft_plan_s* p, int& W, const float2*& hi, const float2*& lo, ui
     return TCFFT_SUCCESS;
 }
 
+// Transforms repeated in place on one buffer grow fp16 values to inf/NaN
+// within two or three transforms (a 65536-point transform scales the rms by
+// 256), and such data runs a few percent faster than finite data (65536 x
+// 256: 28.0 against 29.2-29.7 us).  The tuners therefore start every timed
+// run from a copy of the caller's data, kept here when it can be allocated,
+// and put that data back at the end.
+struct KeptData {
+    void* data;
+    size_t bytes;
+    hipStream_t st;
+    void* keep = nullptr;
+    KeptData(void* d, size_t n, hipStream_t s) : data(d), bytes(n), st(s) {
+        if (hipMalloc(&keep, bytes) != hipSuccess) {
+            keep = nullptr;
+            (void)hipGetLastError();
+        } else if (hipMemcpyAsync(keep, data, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess) {
+            (void)hipGetLastError();
+            (void)hipFree(keep);
+            keep = nullptr;
+        }
+    }
+    KeptData(const KeptData&) = delete;
+    KeptData& operator=(const KeptData&) = delete;
+    int restore() const {
+        if (keep && hipMemcpyAsync(data, keep, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+            return TCFFT_HIP_ERROR;
+        return TCFFT_SUCCESS;
+    }
+    ~KeptData() {
+        if (!keep) return;
+        (void)restore();
+        (void)hipStreamSynchronize(st);
+        (void)hipFree(keep);
+    }
+};
+
 bool aligned16(const void* ptr) { return ptr && !((uintptr_t)ptr & 15u); }
 
 }  // namespace
@@ -1204,12 +1250,15 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
         flush = nullptr;
         (void)hipGetLastError();
     }
-    // mean of five single cold executions after two warm-up ones
+    const KeptData kept(data, plan->total * sizeof(uint32_t), st);
+    // mean of five single cold executions after two warm-up ones, each timed
+    // one on the caller's data
     auto time_current = [&](float& ms) {
         for (int i = 0; i < 2 && !rc; ++i) rc = exec_all(plan, (uint32_t*)data, -1, st);
         float total = 0.f;
         for (int i = 0; i < 5 && !rc; ++i) {
-            if (flush && hipMemsetAsync(flush, i + 1, flush_bytes, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            rc = kept.restore();
+            if (!rc && flush && hipMemsetAsync(flush, i + 1, flush_bytes, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
             if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
             if (!rc) rc = exec_all(plan, (uint32_t*)data, -1, st);
             if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
@@ -1274,6 +1323,8 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
         if (!rc) best_t = std::min(best_t, wg[0].ms);
     }
     // then the tiles per wave of such a pass of lines up to 2048 points
+    // (4096-point lines keep the choice by the grid: their two kernels can
+    // differ in the last bit, and the result should not depend on timing)
     const Pass& p0 = plan->lay.passes[0];
     if (!rc && p0.kind == 0 && p0.radix < 16 && !p0.has_epi) {
         const int saved = plan->rows_tiles;
@@ -1378,7 +1429,7 @@ int exec_natural_staged(tcfft_plan plan, const void* in, void* out, int directio
         return for_chunks(plan, [&](uint32_t s0, uint32_t ns) -> int {
             Pass fp = first;
             fp.n_tiles = (uint32_t)((uint64_t)first.n_tiles / B * ns);
-            tcfft::StartSched sc = fused_rows ? rows_sched(plan, fp) : nat_big_sched(plan, fp.radix);
+            tcfft::StartSched sc = fused_rows ? rows_sched(plan, fp, 16u) : nat_big_sched(plan, fp.radix);
             sc.cached_out = cached_out(plan, 0, ns);
             const uint32_t* ci = rsrc + per * s0;
             uint32_t* co = (uint32_t*)out + per * s0;
@@ -1717,13 +1768,15 @@ int tcfft_multi_autotune(tcfft_multi plan, void* data, void* stream, int* best_g
         flush = nullptr;
         (void)hipGetLastError();
     }
+    const KeptData kept(data, plan->total * sizeof(uint32_t), st);
     // mean of five single timed executions after two warm-up ones, on the
-    // current schedule
+    // current schedule, each timed one on the caller's data
     auto time_current = [&](float& ms) {
         for (int i = 0; i < 2 && !rc; ++i) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
         float total = 0.f;
         for (int i = 0; i < 5 && !rc; ++i) {
-            if (flush && hipMemsetAsync(flush, i + 1, flush_bytes, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
+            rc = kept.restore();
+            if (!rc && flush && hipMemsetAsync(flush, i + 1, flush_bytes, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
             if (!rc && hipEventRecord(a, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
             if (!rc) rc = tcfft_multi_exec(plan, data, TCFFT_FORWARD, stream);
             if (!rc && hipEventRecord(b, st) != hipSuccess) rc = TCFFT_HIP_ERROR;
