@@ -251,9 +251,7 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * meaning two waves per line).  The default is the largest that still gives
  * every SIMD two waves (16 on all but small batches; 16 for 4096-point
  * lines).
- * None of these options changes results, except TCFFT_OPT_ROWS_TILES on
- * 4096-point lines: its two kernels round a few outputs differently in the
- * last bit (same accuracy). */
+ * None of these options changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,
     TCFFT_OPT_BIG_DELAY = 2,

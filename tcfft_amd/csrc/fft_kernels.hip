@@ -197,6 +197,48 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows(u32* __restrict_
 }
 
 // ----------------------------------------------------------------------------
+// Stages 0 and 1 of dit4096 on tiles d2 = 8H..8H+7 (d[t] = tile 8H + t),
+// with its stage-1 recurrence, then the LDS rows [d2][lds_sigma(c)].
+template <int SIGN, int H>
+__device__ __forceinline__ void rows_half_stages(u32 (&d)[8][4], const Dit4096Tw& tw, const Dft16Op& op, u32* xl) {
+    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
+    const int lane = lane_id();
+    const int c16 = lane & 15, g = lane >> 4;
+    cf tw1[4], step[4], rec[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        tw1[i] = cf{tw.tw1[i].x, sg * tw.tw1[i].y};
+        step[i] = cf{tw.step[i].x, sg * tw.step[i].y};
+        rec[i] = step[i];
+    }
+    if constexpr (H == 1) {
+        // rec = W_4096^{8c}: the recurrence's updates after tiles 1..7
+#pragma unroll
+        for (int k = 1; k < 8; ++k)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) rec[i] = cmul(rec[i], step[i]);
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        f32x4 re, im;
+        dft16_a(d[t], op, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
+        dft16_a(d[t], op, re, im);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            cf v = {re[i], im[i]};
+            if (H == 1 || t) {
+                v = cmul(v, rec[i]);
+                if (8 * H + t < 15) rec[i] = cmul(rec[i], step[i]);
+            }
+            d[t][i] = pack(v);
+        }
+        // lane (k1 = c16, k0 = 4g + i): columns 4g + 16*c16 + i
+        store4(xl + (8u * H + (u32)t) * 256u + lds_sigma(16u * c16 + 4u * g), d[t]);
+    }
+}
+
 // k_rows_half: 4096-point lines (no epilogue) with two waves per line, eight
 // tiles each, so that a small batch puts twice the waves, and each wave half
 // the load latency, on the chip.  Wave h of a line loads tiles d2 = 8h..8h+7
@@ -215,7 +257,6 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half(u32* __rest
                                                                    const float2* __restrict__ w4096,
                                                                    StartSched sched) {
     extern __shared__ u32 lds[];
-    constexpr float sg = SIGN < 0 ? 1.f : -1.f;
     const int lane = lane_id();
     const int c16 = lane & 15, g = lane >> 4;
     const u32 wv = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -238,39 +279,12 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half(u32* __rest
     }
     __builtin_amdgcn_sched_barrier(0);
     const Dft16Op op = make_dft16(lane, SIGN);
-    cf tw1[4], step[4], rec[4];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        tw1[i] = cf{tw.tw1[i].x, sg * tw.tw1[i].y};
-        step[i] = cf{tw.step[i].x, sg * tw.step[i].y};
-        rec[i] = step[i];
-    }
-    if (h) {
-        // rec = W_4096^{8c}, by the one-wave kernel's chain of products
-#pragma unroll
-        for (int k = 1; k < 8; ++k)
-#pragma unroll
-            for (int i = 0; i < 4; ++i) rec[i] = cmul(rec[i], step[i]);
-    }
-#pragma unroll
-    for (int t = 0; t < 8; ++t) {
-        f32x4 re, im;
-        dft16_a(d[t], op, re, im);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) d[t][i] = pack(cmul(cf{re[i], im[i]}, tw1[i]));
-        dft16_a(d[t], op, re, im);
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            cf v = {re[i], im[i]};
-            if (h || t) {
-                v = cmul(v, rec[i]);
-                if (t < 7) rec[i] = cmul(rec[i], step[i]);
-            }
-            d[t][i] = pack(v);
-        }
-        // tile d2 = 8h + t: lane (k1 = c16, k0 = 4g + i), columns 4g + 16*c16 + i
-        store4(xl + (8u * h + (u32)t) * 256u + lds_sigma(16u * c16 + 4u * g), d[t]);
-    }
+    // a uniform branch to one compile-time instance per half, each the same
+    // sequence of operations as dit4096's tiles 0..7 or 8..15
+    if (h)
+        rows_half_stages<SIGN, 1>(d, tw, op, xl);
+    else
+        rows_half_stages<SIGN, 0>(d, tw, op, xl);
     __syncthreads();
     u32 v[2][4][4];  // [q][i: point d2 = 4g + i][j: column]
 #pragma unroll

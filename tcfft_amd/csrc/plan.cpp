@@ -1323,8 +1323,7 @@ int tcfft_plan_autotune(tcfft_plan plan, void* data, void* stream, int* best_gro
         if (!rc) best_t = std::min(best_t, wg[0].ms);
     }
     // then the tiles per wave of such a pass of lines up to 2048 points
-    // (4096-point lines keep the choice by the grid: their two kernels can
-    // differ in the last bit, and the result should not depend on timing)
+    // (4096-point lines keep the choice by the grid, see rows_nt)
     const Pass& p0 = plan->lay.passes[0];
     if (!rc && p0.kind == 0 && p0.radix < 16 && !p0.has_epi) {
         const int saved = plan->rows_tiles;

@@ -35,11 +35,6 @@ def rel_l2(t, want):
     return float(np.linalg.norm(got - want) / np.linalg.norm(want))
 
 
-def to_c(t):
-    h = t.cpu().numpy().astype(np.float64)
-    return h[..., 0] + 1j * h[..., 1]
-
-
 # sizes whose first pass is the single-pass kernel of each width, a longer
 # transform whose first pass folds the next pass's twiddle, and contiguous-row
 # first passes (one workgroup per CU and more)
@@ -311,58 +306,40 @@ def test_mall_chunks_bit_identical(case):
     plan.destroy()
 
 
-def rows_kernel(n, batch, tiles):
-    """Which kernel runs a 4096-point row pass: 'half' (two waves per line)
-    for tiles 4 or 8, or by default while one wave per line leaves at most
-    one wave per SIMD; 'one' otherwise (every other line length: 'one')."""
-    if n != 4096 or tiles == 16:
-        return "one"
-    if tiles in (4, 8):
-        return "half"
-    return "half" if batch <= 4 * torch.cuda.get_device_properties(0).multi_processor_count else "one"
-
-
 @pytest.mark.parametrize("n,batch", [(1024, 1024), (256, 300), (2048, 64), (512, 4096), (1 << 18, 3),
                                      (4096, 1024), (4096, 3), (4096, 2048)])
 def test_rows_tiles_per_wave_bit_identical(n, batch):
     """Tiles per wave of a contiguous-row first pass (TCFFT_OPT_ROWS_TILES 0 =
-    by the grid, 4, 8, 16; 2048-point lines take at least 8; a first pass with
-    a folded twiddle, 2^18 split at 1024 points, always 16) never change
-    results, with any workgroup size, and after tcfft_plan_autotune; other
-    values are rejected.  4096-point lines run one kernel per tile count
-    (16: one wave per line; 8: two waves per line with one or two lines per
-    workgroup, an odd line count included): each kernel's results are
-    identical over workgroup sizes and meet the FP64 bound; the two differ
-    in the last bit of a few outputs (rounding of the FP32 twiddle products),
-    and the autotuner keeps the choice by the grid."""
+    by the grid, 4, 8, 16; 2048-point lines take at least 8; 4096-point lines
+    16, or 8 = two waves per line, k_rows_half, with one or two lines per
+    workgroup and an odd line count; a first pass with a folded twiddle, 2^18
+    split at 1024 points, always 16) never change results, with any
+    workgroup size, and after tcfft_plan_autotune (which puts the caller's
+    data back); other values are rejected."""
     note(f"[gpu] rows tiles per wave n={n} batch={batch}")
     x, d = dropin_input(n, batch, n + 7 * batch)
     plan = tcfft_amd.Plan1D(n, batch, first_log2=10) if n > 65536 else tcfft_amd.Plan1D(n, batch)
     if n > 65536:
         xr = x[:, tcfft_amd.gen_rev(n, plan.radices)]
         d = torch.from_numpy(np.ascontiguousarray(np.stack([xr.real, xr.imag], -1).astype(np.float16))).cuda()
-    want = np.fft.fft(x, axis=1)
-    refs = {}  # kernel -> (forward, inverse) of its first run
+    ref = d.clone()
+    plan.exec(ref.data_ptr())
+    torch.cuda.synchronize()
+    assert rel_l2(ref, np.fft.fft(x, axis=1)) <= (1e-3 if n <= 65536 else 1.5e-3)
     for tiles in (16, 8, 4, 0):
         plan.set_option(tcfft_amd.OPT_ROWS_TILES, tiles)
-        kern = rows_kernel(n, batch, tiles)
         for wg in (1, 4):
             plan.set_option(tcfft_amd.OPT_ROWS_WAVES, wg)
-            got = []
             for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
                 t = d.clone()
                 plan.exec(t.data_ptr(), direction)
                 torch.cuda.synchronize()
-                got.append(t)
-            if kern not in refs:
-                refs[kern] = got
-                assert rel_l2(got[0], want) <= (1e-3 if n <= 65536 else 1.5e-3), (tiles, wg)
-            assert torch.equal(got[0], refs[kern][0]), (tiles, wg)
-            assert torch.equal(got[1], refs[kern][1]), (tiles, wg)
-    if len(refs) == 2:
-        a, b = refs["one"][0], refs["half"][0]
-        assert (a != b).any(-1).float().mean().item() < 0.01
-        assert rel_l2(b, to_c(a)) <= 1e-4
+                if direction == tcfft_amd.FORWARD:
+                    assert torch.equal(t, ref), (tiles, wg)
+                else:
+                    if tiles == 16 and wg == 1:
+                        inv_ref = t
+                    assert torch.equal(t, inv_ref), (tiles, wg)
     plan.set_option(tcfft_amd.OPT_ROWS_WAVES, 0)
     plan.set_option(tcfft_amd.OPT_ROWS_TILES, 0)
     scratch = d.clone()
@@ -371,25 +348,24 @@ def test_rows_tiles_per_wave_bit_identical(n, batch):
     t = d.clone()
     plan.exec(t.data_ptr())
     torch.cuda.synchronize()
-    assert torch.equal(t, refs[rows_kernel(n, batch, 0)][0])
+    assert torch.equal(t, ref)
     for bad in (-1, 2, 12, 32):
         with pytest.raises(tcfft_amd.TcfftError):
             plan.set_option(tcfft_amd.OPT_ROWS_TILES, bad)
     plan.destroy()
 
 
-def test_rows_half_cached_first_pass():
+def test_rows_half_cached_first_pass_bit_identical():
     """Two waves per 4096-point line as the first pass of a 2D plan (256 x
     4096 x 3: its output stores go through the caches for the column pass
-    that follows), one or two lines per workgroup, forward and inverse:
-    identical over workgroup sizes, and within 1e-4 (rel-L2) of one wave per
-    line, which the 2D parity tests hold to the FP64 bound."""
+    that follows), one or two lines per workgroup, equal one wave per line,
+    forward and inverse."""
     note("[gpu] rows half, 2D 256 x 4096 x 3")
     rng = np.random.default_rng(4096)
     x = torch.from_numpy(rng.uniform(-1, 1, (3, 256, 4096, 2)).astype(np.float16)).cuda()
     plan = tcfft_amd.Plan2D(256, 4096, 3)
     for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
-        outs = {}
+        outs = []
         for tiles in (16, 8):
             plan.set_option(tcfft_amd.OPT_ROWS_TILES, tiles)
             for wg in (1, 4):
@@ -397,8 +373,7 @@ def test_rows_half_cached_first_pass():
                 t = x.clone()
                 plan.exec(t.data_ptr(), direction)
                 torch.cuda.synchronize()
-                outs.setdefault(tiles, []).append(t)
-        for tiles, ts in outs.items():
-            assert torch.equal(ts[1], ts[0]), (direction, tiles)
-        assert rel_l2(outs[8][0], to_c(outs[16][0])) <= 1e-4, direction
+                outs.append(t)
+        for t in outs[1:]:
+            assert torch.equal(t, outs[0]), direction
     plan.destroy()
