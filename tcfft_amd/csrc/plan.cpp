@@ -450,6 +450,11 @@ bool valid(tcfft_plan p) { return p && p->magic == kMagic; }
 // Start schedule of the single-pass kernel of width W.  Defaults measured on
 // MI355X (3 groups; 2^15 x 512: 26.2 -> 24.0 us with delay 1, 2^16 x 256:
 // 28.9 -> 26.5 us with delay 2, 2^14 x 2048: 53.9 -> 50.4 us with delay 1).
+// Checked again on finite data (input restored before every timed run,
+// 40 graph-replayed execs, medians of 3 rounds), 2^16 x 256: 3 groups with
+// delay 2 27.60 us, delay 3 27.43, 5 groups delay 1 27.46, 7 groups delay 1
+// 27.66, no stagger 28.13; natural order 5 groups delay 3 37.40 us, 7 groups
+// delay 2 37.11, 3 groups delay 2 38.59.
 tcfft::StartSched big_sched(const tcfft_plan_s* p, int W) {
     const int delay = p->big_delay >= 0 ? p->big_delay : (W == 4 || W == 8 ? 1 : 2);
     return tcfft::StartSched{(uint32_t)delay, (uint32_t)p->big_groups, 0u};
