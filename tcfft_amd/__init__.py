@@ -266,8 +266,10 @@ class _Plan:
         _check(lib().tcfft_plan_set_option(self._h, option, value), "tcfft_plan_set_option")
 
     def autotune(self, data_ptr: int, stream: int = 0):
-        """Picks the fastest start schedule of the single-pass kernel by timing
-        forward transforms of the buffer at data_ptr (overwritten).  Returns
+        """Picks the fastest start schedule of the first pass (then its row
+        workgroup size and tiles per wave, and the column-pass delay) by
+        timing forward transforms of the buffer at data_ptr, each from the
+        buffer's original contents, which it holds again on return.  Returns
         (groups, delay, ms)."""
         g, d, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_float()
         _check(lib().tcfft_plan_autotune(self._h, _vp(data_ptr), _vp(stream), ctypes.byref(g), ctypes.byref(d),
