@@ -250,8 +250,8 @@ __device__ __forceinline__ void rows_half_stages(u32 (&d)[8][4], const Dit4096Tw
 // wave h as columns c = j + 4*c16 + 64q + 128h (q = 0, 1) of the 16 points
 // d2 = 4g + i, the operand of the B-form radix 16 over d2; output
 // X[c + 256*k2], k2 = 4g + i, leaves as 16-byte stores of four columns.
-// lines_per_wg lines per workgroup (2 waves each); a line beyond n_lines
-// loads the last line, keeps the barrier and stores nothing.
+// blockDim.x / 128 lines per workgroup (2 waves each); a line beyond
+// n_lines loads the last line, keeps the barrier and stores nothing.
 template <int SIGN, bool CACHED>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half(u32* __restrict__ buf, u32 n_lines,
                                                                    const float2* __restrict__ w4096,

@@ -39,8 +39,9 @@ struct StartSched {
 };
 
 // First pass: contiguous lines of 256*T elements (T = 1, 2, 4, 8, 16), one
-// wave per tiles_per_wave tiles (16; or 8, or 4 when T <= 4, for T < 16
-// without an epilogue), wg_waves (1, 2 or 4) waves per workgroup.
+// wave per tiles_per_wave tiles (16; without an epilogue also 8, or 4 when
+// T <= 4; for T = 16, 8 is two waves per line), wg_waves (1, 2 or 4) waves
+// per workgroup (two-wave lines: 4 = two lines per workgroup, else one).
 hipError_t launch_rows(int T, uint32_t* buf, uint32_t n_tiles, int sign, const float2* w4096,
                        const EpiParams* epi, StartSched sched, uint32_t wg_waves, uint32_t tiles_per_wave,
                        hipStream_t st);
