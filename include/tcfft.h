@@ -249,8 +249,9 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * 256 points per wave of a contiguous-row first pass without a folded
  * twiddle (at least the line's tiles, and 8 or 16 for 4096-point lines, 8
  * meaning two waves per line).  The default is the largest that still gives
- * every SIMD two waves (16 on all but small batches; 16 for 4096-point
- * lines).
+ * every SIMD two waves (16 on all but small batches); for 4096-point lines
+ * 8 while one wave per line gives at most one wave per SIMD, and on
+ * natural-order input 8 except from 4 to 8 lines per CU.
  * None of these options changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,

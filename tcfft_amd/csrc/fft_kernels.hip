@@ -239,6 +239,45 @@ __device__ __forceinline__ void rows_half_stages(u32 (&d)[8][4], const Dit4096Tw
     }
 }
 
+// After the exchange barrier: wave h reads columns c = j + 4*c16 + 64q + 128h
+// (q = 0, 1) of the 16 points d2 = 4g + i from the exchange rows, runs the
+// B-form radix 16 over d2 and, when active, stores X[c + 256*k2] (k2 = 4g +
+// i) at line as 16-byte runs of four columns.
+template <bool CACHED>
+__device__ __forceinline__ void rows_half_finish(const u32* xl, u32 h, const Dft16Op& op, u32* __restrict__ line,
+                                                 bool active) {
+    const int lane = lane_id();
+    const u32 c16 = (u32)lane & 15u, g = (u32)lane >> 4;
+    u32 v[2][4][4];  // [q][i: point d2 = 4g + i][j: column]
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) load4(v[q][i], xl + (4u * g + i) * 256u + lds_sigma(4u * c16 + 64u * q + 128u * h));
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        u32 o[4][4];  // [j][i: k2 = 4g + i]
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            u32 a[4] = {v[q][0][j], v[q][1][j], v[q][2][j], v[q][3][j]};
+            f32x4 re, im;
+            dft16_b(a, op, re, im);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o[j][i] = pack(cf{re[i], im[i]});
+        }
+        if (active) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                u32 s[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
+                u32* const p = line + 256u * (4u * g + i) + 4u * c16 + 64u * q + 128u * h;
+                if constexpr (CACHED)
+                    store4(p, s);
+                else
+                    store4_nt(p, s);
+            }
+        }
+    }
+}
+
 // k_rows_half: 4096-point lines (no epilogue) with two waves per line, eight
 // tiles each, so that a small batch puts twice the waves, and each wave half
 // the load latency, on the chip.  Wave h of a line loads tiles d2 = 8h..8h+7
@@ -286,34 +325,7 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half(u32* __rest
     else
         rows_half_stages<SIGN, 0>(d, tw, op, xl);
     __syncthreads();
-    u32 v[2][4][4];  // [q][i: point d2 = 4g + i][j: column]
-#pragma unroll
-    for (int q = 0; q < 2; ++q)
-#pragma unroll
-        for (int i = 0; i < 4; ++i) load4(v[q][i], xl + (4u * g + i) * 256u + lds_sigma(4u * c16 + 64u * q + 128u * h));
-#pragma unroll
-    for (int q = 0; q < 2; ++q) {
-        u32 o[4][4];  // [j][i: k2 = 4g + i]
-#pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            u32 a[4] = {v[q][0][j], v[q][1][j], v[q][2][j], v[q][3][j]};
-            f32x4 re, im;
-            dft16_b(a, op, re, im);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) o[j][i] = pack(cf{re[i], im[i]});
-        }
-        if (active) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                u32 s[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
-                u32* const p = lbase + 256u * (4u * g + i) + 4u * c16 + 64u * q + 128u * h;
-                if constexpr (CACHED)
-                    store4(p, s);
-                else
-                    store4_nt(p, s);
-            }
-        }
-    }
+    rows_half_finish<CACHED>(xl, h, op, lbase, active);
 }
 
 // ----------------------------------------------------------------------------
@@ -430,6 +442,71 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* i
     }
     auto tile_ptr = [&](int t) -> u32* { return out + (size_t)dst_line[t / T] * L + 256u * ((u32)t % T) + lane_off; };
     rows_body<T, false, SIGN, 16, CACHED>(tile_ptr, ntl, tile0, w4096, EpiParams{}, load);
+}
+
+// k_rows_half on natural-order input (a 4096-point row pass of
+// tcfft_exec_natural, in -> out, in place allowed): the line's two waves
+// load its halves with 16-byte loads into 16 KiB of LDS (word nat_lds_word,
+// as k_rows_nat), and after a barrier wave h reads tiles d2 = 8h..8h+7 of
+// the digit-reversed layout (word y_j ^ d2, see nat_rows_read); the rest is
+// k_rows_half's, with the exchange rows in a second 16 KiB.  Every load of
+// a line precedes the barrier before its first store, so in place is safe.
+// 2D (nx_log2 > 0): input line img*nx + x goes to line img*nx + xpos[x].
+template <int SIGN, bool CACHED>
+__global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half_nat(const u32* in, u32* out, u32 n_lines,
+                                                                       const float2* __restrict__ w4096,
+                                                                       u32 nx_log2, const u32* __restrict__ xpos,
+                                                                       StartSched sched) {
+    extern __shared__ u32 lds[];
+    const int lane = lane_id();
+    const u32 c16 = (u32)lane & 15u, g = (u32)lane >> 4;
+    const u32 wv = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 h = wv & 1u, ln = wv >> 1;
+    const u32 line = blockIdx.x * (blockDim.x >> 7) + ln;
+    const bool active = line < n_lines;
+    start_delay(sched);
+    const u32 src_line = active ? line : n_lines - 1u;
+    u32 dst_line = src_line;
+    if (nx_log2) {
+        const u32 mask = (1u << nx_log2) - 1u;
+        dst_line = (src_line & ~mask) | xpos[src_line & mask];
+    }
+    dst_line = (u32)__builtin_amdgcn_readfirstlane((int)dst_line);
+    u32* const xin = lds + ln * 8192u;  // the natural-order line
+    u32* const xl = xin + 4096u;        // the exchange rows
+
+    Dit4096Tw tw;
+    dit4096_tw_load(tw, lane, w4096);
+    __builtin_amdgcn_sched_barrier(0);
+    {
+        const u32* const src = in + (size_t)src_line * 4096u + 2048u * h + 4u * (u32)lane;
+        u32 v[8][4];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) load4_nt(v[t], src + 256u * t);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) store4(xin + nat_lds_word(2048u * h + 256u * t + 4u * (u32)lane), v[t]);
+    }
+    __syncthreads();
+    u32 d[8][4];
+    {
+        u32 y[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const u32 lp = 16u * c16 + 256u * (4u * g + (u32)j);
+            y[j] = lp ^ (((lp >> 6) & 3u) * 0x14u);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) d[t][j] = xin[y[j] ^ (8u * h + (u32)t)];
+    }
+    const Dft16Op op = make_dft16(lane, SIGN);
+    if (h)
+        rows_half_stages<SIGN, 1>(d, tw, op, xl);
+    else
+        rows_half_stages<SIGN, 0>(d, tw, op, xl);
+    __syncthreads();
+    rows_half_finish<CACHED>(xl, h, op, out + (size_t)dst_line * 4096u, active);
 }
 
 // ----------------------------------------------------------------------------
@@ -1769,9 +1846,33 @@ static hipError_t launch_rows_nat_t(const u32* in, u32* out, u32 n_tiles, const 
     return hipGetLastError();
 }
 
+// k_rows_half_nat over n_tiles / 16 lines, one line (two waves, 32 KiB of
+// LDS) per workgroup.
+template <int SIGN, bool CACHED>
+static hipError_t launch_rows_half_nat_t(const u32* in, u32* out, u32 n_tiles, const float2* w4096, u32 nx_log2,
+                                         const u32* xpos, StartSched sched, hipStream_t st) {
+    const u32 lines = n_tiles / 16u;
+    const size_t lds = 8192u * sizeof(u32);
+    auto k = k_rows_half_nat<SIGN, CACHED>;
+    const KernelInfo ki = kernel_info(k, 128, lds);
+    if (ki.lds_status != hipSuccess) return ki.lds_status;
+    if (sched.groups == 0) sched.delay = 0;
+    if (sched.delay) sched.resident = ki.resident;
+    hipLaunchKernelGGL(k, dim3(lines), dim3(128), lds, st, in, out, lines, w4096, nx_log2, xpos, sched);
+    return hipGetLastError();
+}
+
 hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign, const float2* w4096, u32 nx_log2,
-                           const u32* xpos, StartSched sched, hipStream_t st) {
+                           const u32* xpos, StartSched sched, u32 tiles_per_wave, hipStream_t st) {
     if (nx_log2 && !xpos) return hipErrorInvalidValue;
+    if (tiles_per_wave == 8 && T == 16) {
+        if (sched.cached_out)
+            return sign < 0 ? launch_rows_half_nat_t<-1, true>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st)
+                            : launch_rows_half_nat_t<+1, true>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+        return sign < 0 ? launch_rows_half_nat_t<-1, false>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st)
+                        : launch_rows_half_nat_t<+1, false>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    }
+    if (tiles_per_wave != 16) return hipErrorInvalidValue;
     switch (T * 2 + (sign < 0 ? 0 : 1)) {
     case 2: return launch_rows_nat_t<1, -1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
     case 3: return launch_rows_nat_t<1, +1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);

@@ -128,9 +128,11 @@ struct RevSpec {
 // First pass of k_rows<T> on natural-order input, in -> out (in place allowed
 // when nx_log2 = 0).  2D (nx_log2 > 0, out of place): input line img*nx + x is
 // written to line img*nx + xpos[x] (xpos: device table of nx words, the digit
-// reversal over the x radices in reverse order).
+// reversal over the x radices in reverse order).  tiles_per_wave: 16, or 8
+// for T = 16 (two waves per line).
 hipError_t launch_rows_nat(int T, const uint32_t* in, uint32_t* out, uint32_t n_tiles, int sign, const float2* w4096,
-                           uint32_t nx_log2, const uint32_t* xpos, StartSched sched, hipStream_t st);
+                           uint32_t nx_log2, const uint32_t* xpos, StartSched sched, uint32_t tiles_per_wave,
+                           hipStream_t st);
 
 // Real-input transforms of N = 512*T points (T = 1, 2, 4, 8, 16), n_tiles =
 // batch * T.  sign -1: real in (batch * N/2 words, 16-byte aligned) -> bins

@@ -518,6 +518,19 @@ uint32_t rows_nt(const tcfft_plan_s* p, const Pass& ps) {
     return lo;
 }
 
+// The same for the natural-order pass of 4096-point lines (k_rows_nat<16>
+// or k_rows_half_nat), whose lines go through LDS whole before any compute:
+// two waves per line except from 4*cus to 8*cus lines.  Measured (out of
+// place, random input; one / two waves per line): 4096 x 256 8.1 / 5.95 us,
+// 4096 x 1024 10.6 / 8.5, 4096 x 1536 13.1 / 13.7, 4096 x 2048 15.2 /
+// 16.2, 4096 x 4096 30.7 / 27.2, 4096 x 16384 101 / 93; 2D 4 x 4096 x 256
+// 22.5 / 20.4, 8 x 4096 x 256 32.3 / 31.3.
+uint32_t rows_nt_nat(const tcfft_plan_s* p, const Pass& ps) {
+    if (p->rows_tiles > 0) return p->rows_tiles == 16 ? 16u : 8u;
+    const uint32_t lines = ps.n_tiles / 16u, cus = (uint32_t)p->cus;
+    return lines > 4u * cus && lines <= 8u * cus ? 16u : 8u;
+}
+
 // Start schedule of a contiguous-row first pass run with nt tiles per wave
 // (the natural-order kernel k_rows_nat always runs 16).
 tcfft::StartSched rows_sched(const tcfft_plan_s* p, const Pass& ps, uint32_t nt) {
@@ -1433,12 +1446,15 @@ int exec_natural_staged(tcfft_plan plan, const void* in, void* out, int directio
         return for_chunks(plan, [&](uint32_t s0, uint32_t ns) -> int {
             Pass fp = first;
             fp.n_tiles = (uint32_t)((uint64_t)first.n_tiles / B * ns);
-            tcfft::StartSched sc = fused_rows ? rows_sched(plan, fp, 16u) : nat_big_sched(plan, fp.radix);
+            // 4096-point lines: one or two waves per line (rows_nt_nat);
+            // shorter lines 16 tiles per wave
+            const uint32_t nt = fused_rows && fp.radix == 16 ? rows_nt_nat(plan, fp) : 16u;
+            tcfft::StartSched sc = fused_rows ? rows_sched(plan, fp, nt) : nat_big_sched(plan, fp.radix);
             sc.cached_out = cached_out(plan, 0, ns);
             const uint32_t* ci = rsrc + per * s0;
             uint32_t* co = (uint32_t*)out + per * s0;
             hipError_t e = fused_rows ? tcfft::launch_rows_nat(fp.radix, ci, co, fp.n_tiles, direction, plan->w4096,
-                                                               nx_log2, plan->xpos, sc, st)
+                                                               nx_log2, plan->xpos, sc, nt, st)
                                       : tcfft::launch_big_nat(fp.radix, ci, co, fp.n_tiles, direction, plan->w4096,
                                                               fp.big_hi, fp.big_lo, fp.big_M_log2 / 2, nx_log2, x_pos,
                                                               sc, st);

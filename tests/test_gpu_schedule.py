@@ -377,3 +377,39 @@ def test_rows_half_cached_first_pass_bit_identical():
         for t in outs[1:]:
             assert torch.equal(t, outs[0]), direction
     plan.destroy()
+
+
+@pytest.mark.parametrize("shape", [(4096, 0, 5), (4096, 0, 1024), (8, 4096, 3)], ids=["1d-4096x5", "1d-4096x1024",
+                                                                                   "2d-8x4096x3"])
+def test_rows_half_natural_bit_identical(shape):
+    """Natural-order input on 4096-point lines with two waves per line
+    (k_rows_half_nat: TCFFT_OPT_ROWS_TILES 8, the default for these grids)
+    equals one wave per line (16) and the drop-in path on host-permuted
+    input, bit for bit: 1D out of place and in place, 2D out of place and in
+    place (through the plan's staging copy), forward and inverse."""
+    a, b, batch = shape
+    note(f"[gpu] rows half natural {shape}")
+    rng = np.random.default_rng(a + b + batch)
+    if b:
+        plan = tcfft_amd.Plan2D(a, b, batch)
+        x = rng.uniform(-1, 1, (batch, a, b, 2)).astype(np.float16)
+        xr = x[:, tcfft_amd.gen_rev(a, plan.radices_x)][:, :, tcfft_amd.gen_rev(b, plan.radices_y)]
+    else:
+        plan = tcfft_amd.Plan1D(a, batch)
+        x = rng.uniform(-1, 1, (batch, a, 2)).astype(np.float16)
+        xr = x[:, tcfft_amd.gen_rev(a, plan.radices)]
+    src = torch.from_numpy(x).cuda()
+    for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
+        want = torch.from_numpy(np.ascontiguousarray(xr)).cuda()
+        plan.exec(want.data_ptr(), direction)
+        for tiles in (16, 8, 0):
+            plan.set_option(tcfft_amd.OPT_ROWS_TILES, tiles)
+            out = torch.empty_like(src)
+            plan.exec_natural(src.data_ptr(), out.data_ptr(), direction)
+            inp = src.clone()
+            plan.exec_natural(inp.data_ptr(), inp.data_ptr(), direction)
+            torch.cuda.synchronize()
+            assert torch.equal(out, want), (tiles, direction)
+            assert torch.equal(inp, want), (tiles, direction)
+    plan.set_option(tcfft_amd.OPT_ROWS_TILES, 0)
+    plan.destroy()
