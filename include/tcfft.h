@@ -247,11 +247,11 @@ int tcfft_conv_filter_grad(tcfft_plan plan, int channels, const void *x_spec, co
  * whose single signal exceeds the size are not chunked.
  * TCFFT_OPT_ROWS_TILES (4, 8 or 16; 0, the default, = by the grid): tiles of
  * 256 points per wave of a contiguous-row first pass without a folded
- * twiddle (at least the line's tiles, and 8 or 16 for 4096-point lines, 8
- * meaning two waves per line).  The default is the largest that still gives
+ * twiddle (at least the line's tiles; for 4096-point lines 8 and 4 mean two
+ * and four waves per line).  The default is the largest that still gives
  * every SIMD two waves (16 on all but small batches); for 4096-point lines
- * 8 while one wave per line gives at most one wave per SIMD, and on
- * natural-order input 8 except from 4 to 8 lines per CU.
+ * 4 up to one line per CU, 8 up to four, 16 above, and on natural-order
+ * input 4 up to one line per CU, 16 from 4 to 8 lines per CU, 8 otherwise.
  * None of these options changes results. */
 enum tcfft_option {
     TCFFT_OPT_SPLIT_CHUNK = 1,

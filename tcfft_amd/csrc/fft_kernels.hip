@@ -197,10 +197,14 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows(u32* __restrict_
 }
 
 // ----------------------------------------------------------------------------
-// Stages 0 and 1 of dit4096 on tiles d2 = 8H..8H+7 (d[t] = tile 8H + t),
-// with its stage-1 recurrence, then the LDS rows [d2][lds_sigma(c)].
-template <int SIGN, int H>
-__device__ __forceinline__ void rows_half_stages(u32 (&d)[8][4], const Dit4096Tw& tw, const Dft16Op& op, u32* xl) {
+// Stages 0 and 1 of dit4096 on tiles d2 = TPW*H .. TPW*H + TPW-1 (d[t] =
+// tile TPW*H + t), with its stage-1 recurrence, then the LDS rows
+// [d2][lds_sigma(c)].  A later part's recurrence is first advanced by the
+// updates dit4096 makes after tiles 1 .. TPW*H - 1, so that every product,
+// in order, is dit4096's.
+template <int SIGN, int TPW, int H>
+__device__ __forceinline__ void rows_split_stages(u32 (&d)[TPW][4], const Dit4096Tw& tw, const Dft16Op& op,
+                                                  u32* xl) {
     constexpr float sg = SIGN < 0 ? 1.f : -1.f;
     const int lane = lane_id();
     const int c16 = lane & 15, g = lane >> 4;
@@ -211,15 +215,13 @@ __device__ __forceinline__ void rows_half_stages(u32 (&d)[8][4], const Dit4096Tw
         step[i] = cf{tw.step[i].x, sg * tw.step[i].y};
         rec[i] = step[i];
     }
-    if constexpr (H == 1) {
-        // rec = W_4096^{8c}: the recurrence's updates after tiles 1..7
 #pragma unroll
-        for (int k = 1; k < 8; ++k)
+    for (int k = 1; k < TPW * H; ++k)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) rec[i] = cmul(rec[i], step[i]);
-    }
+        for (int i = 0; i < 4; ++i) rec[i] = cmul(rec[i], step[i]);
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < TPW; ++t) {
+        constexpr int d2_0 = TPW * H;
         f32x4 re, im;
         dft16_a(d[t], op, re, im);
 #pragma unroll
@@ -228,33 +230,57 @@ __device__ __forceinline__ void rows_half_stages(u32 (&d)[8][4], const Dit4096Tw
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             cf v = {re[i], im[i]};
-            if (H == 1 || t) {
+            if (d2_0 + t > 0) {
                 v = cmul(v, rec[i]);
-                if (8 * H + t < 15) rec[i] = cmul(rec[i], step[i]);
+                if (d2_0 + t < 15) rec[i] = cmul(rec[i], step[i]);
             }
             d[t][i] = pack(v);
         }
         // lane (k1 = c16, k0 = 4g + i): columns 4g + 16*c16 + i
-        store4(xl + (8u * H + (u32)t) * 256u + lds_sigma(16u * c16 + 4u * g), d[t]);
+        store4(xl + (u32)(d2_0 + t) * 256u + lds_sigma(16u * c16 + 4u * g), d[t]);
     }
 }
 
-// After the exchange barrier: wave h reads columns c = j + 4*c16 + 64q + 128h
-// (q = 0, 1) of the 16 points d2 = 4g + i from the exchange rows, runs the
-// B-form radix 16 over d2 and, when active, stores X[c + 256*k2] (k2 = 4g +
-// i) at line as 16-byte runs of four columns.
-template <bool CACHED>
-__device__ __forceinline__ void rows_half_finish(const u32* xl, u32 h, const Dft16Op& op, u32* __restrict__ line,
-                                                 bool active) {
+// Part h (wave-uniform) of a line split over WPL waves: a uniform branch to
+// the compile-time instance of rows_split_stages.
+template <int SIGN, int WPL>
+__device__ __forceinline__ void rows_split_part(u32 h, u32 (&d)[16 / WPL][4], const Dit4096Tw& tw, const Dft16Op& op,
+                                                u32* xl) {
+    static_assert(WPL == 2 || WPL == 4, "two or four waves per line");
+    constexpr int TPW = 16 / WPL;
+    if constexpr (WPL == 2) {
+        if (h)
+            rows_split_stages<SIGN, TPW, 1>(d, tw, op, xl);
+        else
+            rows_split_stages<SIGN, TPW, 0>(d, tw, op, xl);
+    } else {
+        switch (h) {
+        case 0: rows_split_stages<SIGN, TPW, 0>(d, tw, op, xl); break;
+        case 1: rows_split_stages<SIGN, TPW, 1>(d, tw, op, xl); break;
+        case 2: rows_split_stages<SIGN, TPW, 2>(d, tw, op, xl); break;
+        default: rows_split_stages<SIGN, TPW, 3>(d, tw, op, xl); break;
+        }
+    }
+}
+
+// After the exchange barrier: part h reads columns c = j + 4*c16 + 64q +
+// (256/WPL) h (q < 4/WPL) of the 16 points d2 = 4g + i from the exchange
+// rows, runs the B-form radix 16 over d2 and, when active, stores X[c +
+// 256*k2] (k2 = 4g + i) at line as 16-byte runs of four columns.
+template <bool CACHED, int WPL>
+__device__ __forceinline__ void rows_split_finish(const u32* xl, u32 h, const Dft16Op& op, u32* __restrict__ line,
+                                                  bool active) {
+    constexpr int QN = 4 / WPL;
     const int lane = lane_id();
     const u32 c16 = (u32)lane & 15u, g = (u32)lane >> 4;
-    u32 v[2][4][4];  // [q][i: point d2 = 4g + i][j: column]
+    const u32 cb = (256u / WPL) * h;
+    u32 v[QN][4][4];  // [q][i: point d2 = 4g + i][j: column]
 #pragma unroll
-    for (int q = 0; q < 2; ++q)
+    for (int q = 0; q < QN; ++q)
 #pragma unroll
-        for (int i = 0; i < 4; ++i) load4(v[q][i], xl + (4u * g + i) * 256u + lds_sigma(4u * c16 + 64u * q + 128u * h));
+        for (int i = 0; i < 4; ++i) load4(v[q][i], xl + (4u * g + i) * 256u + lds_sigma(4u * c16 + 64u * q + cb));
 #pragma unroll
-    for (int q = 0; q < 2; ++q) {
+    for (int q = 0; q < QN; ++q) {
         u32 o[4][4];  // [j][i: k2 = 4g + i]
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -268,7 +294,7 @@ __device__ __forceinline__ void rows_half_finish(const u32* xl, u32 h, const Dft
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 u32 s[4] = {o[0][i], o[1][i], o[2][i], o[3][i]};
-                u32* const p = line + 256u * (4u * g + i) + 4u * c16 + 64u * q + 128u * h;
+                u32* const p = line + 256u * (4u * g + i) + 4u * c16 + 64u * q + cb;
                 if constexpr (CACHED)
                     store4(p, s);
                 else
@@ -278,29 +304,29 @@ __device__ __forceinline__ void rows_half_finish(const u32* xl, u32 h, const Dft
     }
 }
 
-// k_rows_half: 4096-point lines (no epilogue) with two waves per line, eight
-// tiles each, so that a small batch puts twice the waves, and each wave half
-// the load latency, on the chip.  Wave h of a line loads tiles d2 = 8h..8h+7
-// in k_rows<16>'s layout and runs stages 0 and 1 of dit4096 on them, with
-// the stage-1 recurrence W_4096^{c*d2} advanced over tiles 1..8h-1 first, so
-// that every product equals the one-wave kernel's (bit-identical results).
-// The stage-2 inputs go through LDS, [d2][lds_sigma(c)] per line (16 KiB):
-// written as 16-byte rows of four columns c = 4g + i + 16*c16, read back by
-// wave h as columns c = j + 4*c16 + 64q + 128h (q = 0, 1) of the 16 points
-// d2 = 4g + i, the operand of the B-form radix 16 over d2; output
+// k_rows_half<SIGN, CACHED, WPL>: 4096-point lines (no epilogue) split over
+// WPL = 2 or 4 waves, 16/WPL tiles each, so that a small batch puts WPL times
+// the waves, each with 1/WPL of the load latency, on the chip.  Part h of a
+// line loads tiles d2 = (16/WPL) h .. in k_rows<16>'s layout and runs
+// stages 0 and 1 of dit4096 on them (rows_split_stages: bit-identical
+// results).  The stage-2 inputs go through LDS, [d2][lds_sigma(c)] per line
+// (16 KiB): written as 16-byte rows of four columns c = 4g + i + 16*c16,
+// read back by part h as columns c = j + 4*c16 + 64q + (256/WPL) h of the
+// 16 points d2 = 4g + i, the operand of the B-form radix 16 over d2; output
 // X[c + 256*k2], k2 = 4g + i, leaves as 16-byte stores of four columns.
-// blockDim.x / 128 lines per workgroup (2 waves each); a line beyond
-// n_lines loads the last line, keeps the barrier and stores nothing.
-template <int SIGN, bool CACHED>
+// blockDim.x / (64 WPL) lines per workgroup; a line beyond n_lines loads the
+// last line, keeps the barrier and stores nothing.
+template <int SIGN, bool CACHED, int WPL>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half(u32* __restrict__ buf, u32 n_lines,
                                                                    const float2* __restrict__ w4096,
                                                                    StartSched sched) {
+    constexpr int TPW = 16 / WPL;
     extern __shared__ u32 lds[];
     const int lane = lane_id();
     const int c16 = lane & 15, g = lane >> 4;
     const u32 wv = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const u32 h = wv & 1u, ln = wv >> 1;
-    const u32 line = blockIdx.x * (blockDim.x >> 7) + ln;
+    const u32 h = wv % WPL, ln = wv / WPL;
+    const u32 line = blockIdx.x * (blockDim.x / (64u * WPL)) + ln;
     const bool active = line < n_lines;
     start_delay(sched);
     u32* const lbase = buf + (size_t)(active ? line : n_lines - 1u) * 4096u;
@@ -309,23 +335,18 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half(u32* __rest
     Dit4096Tw tw;
     dit4096_tw_load(tw, lane, w4096);
     __builtin_amdgcn_sched_barrier(0);
-    u32 d[8][4];
-    u32* const tin = lbase + 2048u * h + 16u * c16 + 4u * g;
+    u32 d[TPW][4];
+    u32* const tin = lbase + (4096u / WPL) * h + 16u * c16 + 4u * g;
 #pragma unroll
-    for (int t = 0; t < 8; ++t) {
+    for (int t = 0; t < TPW; ++t) {
         load4_nt(d[t], tin + 256 * t);
         __builtin_amdgcn_sched_barrier(0);
     }
     __builtin_amdgcn_sched_barrier(0);
     const Dft16Op op = make_dft16(lane, SIGN);
-    // a uniform branch to one compile-time instance per half, each the same
-    // sequence of operations as dit4096's tiles 0..7 or 8..15
-    if (h)
-        rows_half_stages<SIGN, 1>(d, tw, op, xl);
-    else
-        rows_half_stages<SIGN, 0>(d, tw, op, xl);
+    rows_split_part<SIGN, WPL>(h, d, tw, op, xl);
     __syncthreads();
-    rows_half_finish<CACHED>(xl, h, op, lbase, active);
+    rows_split_finish<CACHED, WPL>(xl, h, op, lbase, active);
 }
 
 // ----------------------------------------------------------------------------
@@ -445,24 +466,25 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_nat(const u32* i
 }
 
 // k_rows_half on natural-order input (a 4096-point row pass of
-// tcfft_exec_natural, in -> out, in place allowed): the line's two waves
-// load its halves with 16-byte loads into 16 KiB of LDS (word nat_lds_word,
-// as k_rows_nat), and after a barrier wave h reads tiles d2 = 8h..8h+7 of
-// the digit-reversed layout (word y_j ^ d2, see nat_rows_read); the rest is
-// k_rows_half's, with the exchange rows in a second 16 KiB.  Every load of
-// a line precedes the barrier before its first store, so in place is safe.
-// 2D (nx_log2 > 0): input line img*nx + x goes to line img*nx + xpos[x].
-template <int SIGN, bool CACHED>
+// tcfft_exec_natural, in -> out, in place allowed): the line's WPL waves
+// load its parts with 16-byte loads into 16 KiB of LDS (word nat_lds_word,
+// as k_rows_nat), and after a barrier part h reads tiles d2 = (16/WPL) h ..
+// of the digit-reversed layout (word y_j ^ d2, see nat_rows_read); the rest
+// is k_rows_half's, with the exchange rows in a second 16 KiB.  Every load
+// of a line precedes the barrier before its first store, so in place is
+// safe.  2D (nx_log2 > 0): input line img*nx + x goes to line img*nx +
+// xpos[x].  One line per workgroup.
+template <int SIGN, bool CACHED, int WPL>
 __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half_nat(const u32* in, u32* out, u32 n_lines,
                                                                        const float2* __restrict__ w4096,
                                                                        u32 nx_log2, const u32* __restrict__ xpos,
                                                                        StartSched sched) {
+    constexpr int TPW = 16 / WPL;
     extern __shared__ u32 lds[];
     const int lane = lane_id();
     const u32 c16 = (u32)lane & 15u, g = (u32)lane >> 4;
-    const u32 wv = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
-    const u32 h = wv & 1u, ln = wv >> 1;
-    const u32 line = blockIdx.x * (blockDim.x >> 7) + ln;
+    const u32 h = (u32)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const u32 line = blockIdx.x;
     const bool active = line < n_lines;
     start_delay(sched);
     const u32 src_line = active ? line : n_lines - 1u;
@@ -472,22 +494,23 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half_nat(const u
         dst_line = (src_line & ~mask) | xpos[src_line & mask];
     }
     dst_line = (u32)__builtin_amdgcn_readfirstlane((int)dst_line);
-    u32* const xin = lds + ln * 8192u;  // the natural-order line
-    u32* const xl = xin + 4096u;        // the exchange rows
+    u32* const xin = lds;          // the natural-order line
+    u32* const xl = lds + 4096u;   // the exchange rows
 
     Dit4096Tw tw;
     dit4096_tw_load(tw, lane, w4096);
     __builtin_amdgcn_sched_barrier(0);
     {
-        const u32* const src = in + (size_t)src_line * 4096u + 2048u * h + 4u * (u32)lane;
-        u32 v[8][4];
+        const u32 part = (4096u / WPL) * h;
+        const u32* const src = in + (size_t)src_line * 4096u + part + 4u * (u32)lane;
+        u32 v[TPW][4];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) load4_nt(v[t], src + 256u * t);
+        for (int t = 0; t < TPW; ++t) load4_nt(v[t], src + 256u * t);
 #pragma unroll
-        for (int t = 0; t < 8; ++t) store4(xin + nat_lds_word(2048u * h + 256u * t + 4u * (u32)lane), v[t]);
+        for (int t = 0; t < TPW; ++t) store4(xin + nat_lds_word(part + 256u * t + 4u * (u32)lane), v[t]);
     }
     __syncthreads();
-    u32 d[8][4];
+    u32 d[TPW][4];
     {
         u32 y[4];
 #pragma unroll
@@ -496,17 +519,14 @@ __global__ __launch_bounds__(256) TCFFT_MFMA_BUDGET void k_rows_half_nat(const u
             y[j] = lp ^ (((lp >> 6) & 3u) * 0x14u);
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
+        for (int t = 0; t < TPW; ++t)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) d[t][j] = xin[y[j] ^ (8u * h + (u32)t)];
+            for (int j = 0; j < 4; ++j) d[t][j] = xin[y[j] ^ ((u32)TPW * h + (u32)t)];
     }
     const Dft16Op op = make_dft16(lane, SIGN);
-    if (h)
-        rows_half_stages<SIGN, 1>(d, tw, op, xl);
-    else
-        rows_half_stages<SIGN, 0>(d, tw, op, xl);
+    rows_split_part<SIGN, WPL>(h, d, tw, op, xl);
     __syncthreads();
-    rows_half_finish<CACHED>(xl, h, op, out + (size_t)dst_line * 4096u, active);
+    rows_split_finish<CACHED, WPL>(xl, h, op, out + (size_t)dst_line * 4096u, active);
 }
 
 // ----------------------------------------------------------------------------
@@ -1735,37 +1755,45 @@ static hipError_t launch_rows_t(u32* buf, u32 n_tiles, const float2* w4096, cons
     return hipGetLastError();
 }
 
-// k_rows_half over n_tiles / 16 lines: wg = 4 puts two lines in a workgroup,
-// 1 or 2 one line.
-template <int SIGN, bool CACHED>
+// k_rows_half<WPL> over n_tiles / 16 lines: for WPL = 2, wg = 4 puts two
+// lines in a workgroup, 1 or 2 one line; for WPL = 4 one line.
+template <int SIGN, bool CACHED, int WPL>
 static hipError_t launch_rows_half_t(u32* buf, u32 n_tiles, const float2* w4096, StartSched sched, u32 wg,
                                      hipStream_t st) {
-    const u32 lines = n_tiles / 16u, lpw = wg == 4u ? 2u : 1u;
+    const u32 lines = n_tiles / 16u, lpw = WPL == 2 && wg == 4u ? 2u : 1u;
     const size_t lds = (size_t)lpw * 4096u * sizeof(u32);
-    auto k = k_rows_half<SIGN, CACHED>;
-    const KernelInfo ki = kernel_info(k, (int)(128 * lpw), lds);
+    auto k = k_rows_half<SIGN, CACHED, WPL>;
+    const KernelInfo ki = kernel_info(k, (int)(64 * WPL * lpw), lds);
     if (ki.lds_status != hipSuccess) return ki.lds_status;
     if (sched.groups == 0) sched.delay = 0;
     if (sched.delay) sched.resident = ki.resident;
-    hipLaunchKernelGGL(k, dim3((lines + lpw - 1u) / lpw), dim3(128 * lpw), lds, st, buf, lines, w4096, sched);
+    hipLaunchKernelGGL(k, dim3((lines + lpw - 1u) / lpw), dim3(64 * WPL * lpw), lds, st, buf, lines, w4096, sched);
     return hipGetLastError();
+}
+
+template <int WPL>
+static hipError_t launch_rows_half_w(u32* buf, u32 n_tiles, int sign, const float2* w4096, StartSched sched, u32 wg,
+                                     hipStream_t st) {
+    if (sign < 0)
+        return sched.cached_out ? launch_rows_half_t<-1, true, WPL>(buf, n_tiles, w4096, sched, wg, st)
+                                : launch_rows_half_t<-1, false, WPL>(buf, n_tiles, w4096, sched, wg, st);
+    return sched.cached_out ? launch_rows_half_t<+1, true, WPL>(buf, n_tiles, w4096, sched, wg, st)
+                            : launch_rows_half_t<+1, false, WPL>(buf, n_tiles, w4096, sched, wg, st);
 }
 
 // Only the 4096-point lines (T = 16) have non-temporal output stores to
 // switch; shorter lines always store through the caches.  Tiles per wave:
 // 16, or 8 / 4 (at least T) for lines of up to 2048 points without an
 // epilogue, which puts more waves on a small grid; for 4096-point lines 8
-// tiles per wave is k_rows_half, two waves per line.
+// or 4 tiles per wave is k_rows_half, two or four waves per line.
 template <int T>
 static hipError_t launch_rows_t(u32* buf, u32 n_tiles, int sign, const float2* w4096, const EpiParams* epi,
                                 StartSched sched, u32 wg, u32 tiles, hipStream_t st) {
     if constexpr (T == 16) {
-        if (tiles == 8) {
+        if (tiles == 8 || tiles == 4) {
             if (epi) return hipErrorInvalidValue;  // the epilogue pass runs 16 tiles per wave only
-            return sign < 0 ? (sched.cached_out ? launch_rows_half_t<-1, true>(buf, n_tiles, w4096, sched, wg, st)
-                                                : launch_rows_half_t<-1, false>(buf, n_tiles, w4096, sched, wg, st))
-                            : (sched.cached_out ? launch_rows_half_t<+1, true>(buf, n_tiles, w4096, sched, wg, st)
-                                                : launch_rows_half_t<+1, false>(buf, n_tiles, w4096, sched, wg, st));
+            return tiles == 8 ? launch_rows_half_w<2>(buf, n_tiles, sign, w4096, sched, wg, st)
+                              : launch_rows_half_w<4>(buf, n_tiles, sign, w4096, sched, wg, st);
         }
         if (tiles != 16) return hipErrorInvalidValue;
         if (sched.cached_out)
@@ -1846,32 +1874,39 @@ static hipError_t launch_rows_nat_t(const u32* in, u32* out, u32 n_tiles, const 
     return hipGetLastError();
 }
 
-// k_rows_half_nat over n_tiles / 16 lines, one line (two waves, 32 KiB of
-// LDS) per workgroup.
-template <int SIGN, bool CACHED>
+// k_rows_half_nat<WPL> over n_tiles / 16 lines, one line (WPL waves,
+// 32 KiB of LDS) per workgroup.
+template <int SIGN, bool CACHED, int WPL>
 static hipError_t launch_rows_half_nat_t(const u32* in, u32* out, u32 n_tiles, const float2* w4096, u32 nx_log2,
                                          const u32* xpos, StartSched sched, hipStream_t st) {
     const u32 lines = n_tiles / 16u;
     const size_t lds = 8192u * sizeof(u32);
-    auto k = k_rows_half_nat<SIGN, CACHED>;
-    const KernelInfo ki = kernel_info(k, 128, lds);
+    auto k = k_rows_half_nat<SIGN, CACHED, WPL>;
+    const KernelInfo ki = kernel_info(k, 64 * WPL, lds);
     if (ki.lds_status != hipSuccess) return ki.lds_status;
     if (sched.groups == 0) sched.delay = 0;
     if (sched.delay) sched.resident = ki.resident;
-    hipLaunchKernelGGL(k, dim3(lines), dim3(128), lds, st, in, out, lines, w4096, nx_log2, xpos, sched);
+    hipLaunchKernelGGL(k, dim3(lines), dim3(64 * WPL), lds, st, in, out, lines, w4096, nx_log2, xpos, sched);
     return hipGetLastError();
+}
+
+template <int WPL>
+static hipError_t launch_rows_half_nat_w(const u32* in, u32* out, u32 n_tiles, int sign, const float2* w4096,
+                                         u32 nx_log2, const u32* xpos, StartSched sched, hipStream_t st) {
+    if (sign < 0)
+        return sched.cached_out ? launch_rows_half_nat_t<-1, true, WPL>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st)
+                                : launch_rows_half_nat_t<-1, false, WPL>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
+    return sched.cached_out ? launch_rows_half_nat_t<+1, true, WPL>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st)
+                            : launch_rows_half_nat_t<+1, false, WPL>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
 }
 
 hipError_t launch_rows_nat(int T, const u32* in, u32* out, u32 n_tiles, int sign, const float2* w4096, u32 nx_log2,
                            const u32* xpos, StartSched sched, u32 tiles_per_wave, hipStream_t st) {
     if (nx_log2 && !xpos) return hipErrorInvalidValue;
-    if (tiles_per_wave == 8 && T == 16) {
-        if (sched.cached_out)
-            return sign < 0 ? launch_rows_half_nat_t<-1, true>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st)
-                            : launch_rows_half_nat_t<+1, true>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
-        return sign < 0 ? launch_rows_half_nat_t<-1, false>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st)
-                        : launch_rows_half_nat_t<+1, false>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);
-    }
+    if ((tiles_per_wave == 8 || tiles_per_wave == 4) && T == 16)
+        return tiles_per_wave == 8
+                   ? launch_rows_half_nat_w<2>(in, out, n_tiles, sign, w4096, nx_log2, xpos, sched, st)
+                   : launch_rows_half_nat_w<4>(in, out, n_tiles, sign, w4096, nx_log2, xpos, sched, st);
     if (tiles_per_wave != 16) return hipErrorInvalidValue;
     switch (T * 2 + (sign < 0 ? 0 : 1)) {
     case 2: return launch_rows_nat_t<1, -1>(in, out, n_tiles, w4096, nx_log2, xpos, sched, st);

@@ -40,8 +40,9 @@ struct StartSched {
 
 // First pass: contiguous lines of 256*T elements (T = 1, 2, 4, 8, 16), one
 // wave per tiles_per_wave tiles (16; without an epilogue also 8, or 4 when
-// T <= 4; for T = 16, 8 is two waves per line), wg_waves (1, 2 or 4) waves
-// per workgroup (two-wave lines: 4 = two lines per workgroup, else one).
+// T <= 4; for T = 16, 8 or 4 is two or four waves per line), wg_waves (1,
+// 2 or 4) waves per workgroup (two-wave lines: 4 = two lines per workgroup,
+// else one).
 hipError_t launch_rows(int T, uint32_t* buf, uint32_t n_tiles, int sign, const float2* w4096,
                        const EpiParams* epi, StartSched sched, uint32_t wg_waves, uint32_t tiles_per_wave,
                        hipStream_t st);
@@ -129,7 +130,7 @@ struct RevSpec {
 // when nx_log2 = 0).  2D (nx_log2 > 0, out of place): input line img*nx + x is
 // written to line img*nx + xpos[x] (xpos: device table of nx words, the digit
 // reversal over the x radices in reverse order).  tiles_per_wave: 16, or 8
-// for T = 16 (two waves per line).
+// or 4 for T = 16 (two or four waves per line).
 hipError_t launch_rows_nat(int T, const uint32_t* in, uint32_t* out, uint32_t n_tiles, int sign, const float2* w4096,
                            uint32_t nx_log2, const uint32_t* xpos, StartSched sched, uint32_t tiles_per_wave,
                            hipStream_t st);

@@ -490,12 +490,13 @@ uint32_t rows_wg(const tcfft_plan_s* p, const Pass& ps) {
 }
 
 // Tiles (of 256 points) per wave of a contiguous-row first pass: 16 for
-// passes with an epilogue; for 4096-point lines 16, or 8 (two waves per
-// line, k_rows_half) when the plan sets 4 or 8, or by default when one wave
-// per line gives at most one wave per SIMD (random input, rotated buffers:
-// 4096 x 1024 8.3-8.4 -> 7.8 us, 4096 x 256 5.7 -> 4.9 us; at 2048 lines
-// and more one wave per line is ahead, 12.1 against 14.6 us, and so is 2D
-// 256 x 4096 x 8 with 2048 lines, 27.9 against 28.8); otherwise the plan's setting,
+// passes with an epilogue; for 4096-point lines the plan's setting, or by
+// default 4 (four waves per line, k_rows_half) up to one line per CU, 8
+// (two waves) up to four lines per CU, and 16 above (random input, input
+// restored before every timed run, one / two / four waves per line: 4096 x
+// 256 6.55 / 5.39 / 5.31 us, 4096 x 1024 7.8-8.1 / 7.35 / 9.3, 4096 x 2048
+// 13.9 / 14.2 / 16.0, 4096 x 4096 26.4 / 25.9 / 27.5; 2D 256 x 4096 x 8,
+// 2048 lines, 27.9 / 28.8 on inf/NaN data); otherwise the plan's setting,
 // or by default the largest of 16, 8, 4 (at least the line's tiles) that
 // still gives every SIMD two waves, so that a small grid spreads over the
 // chip.  Measured (random input; 16 / 8 / 4 tiles): 1024 x 1024 6.17 / 5.23
@@ -507,8 +508,9 @@ uint32_t rows_nt(const tcfft_plan_s* p, const Pass& ps) {
     const uint32_t T = (uint32_t)ps.radix;
     if (ps.has_epi) return 16u;
     if (T >= 16) {
-        if (p->rows_tiles > 0) return p->rows_tiles == 16 ? 16u : 8u;
-        return ps.n_tiles / 16u <= 4u * (uint32_t)p->cus ? 8u : 16u;
+        if (p->rows_tiles > 0) return (uint32_t)p->rows_tiles;
+        const uint32_t lines = ps.n_tiles / 16u, cus = (uint32_t)p->cus;
+        return lines <= cus ? 4u : lines <= 4u * cus ? 8u : 16u;
     }
     const uint32_t lo = T <= 4 ? 4u : 8u;
     if (p->rows_tiles > 0) return std::max((uint32_t)p->rows_tiles, lo);
@@ -520,14 +522,16 @@ uint32_t rows_nt(const tcfft_plan_s* p, const Pass& ps) {
 
 // The same for the natural-order pass of 4096-point lines (k_rows_nat<16>
 // or k_rows_half_nat), whose lines go through LDS whole before any compute:
-// two waves per line except from 4*cus to 8*cus lines.  Measured (out of
-// place, random input; one / two waves per line): 4096 x 256 8.1 / 5.95 us,
-// 4096 x 1024 10.6 / 8.5, 4096 x 1536 13.1 / 13.7, 4096 x 2048 15.2 /
-// 16.2, 4096 x 4096 30.7 / 27.2, 4096 x 16384 101 / 93; 2D 4 x 4096 x 256
-// 22.5 / 20.4, 8 x 4096 x 256 32.3 / 31.3.
+// four waves per line up to one line per CU, one wave from 4 to 8 lines
+// per CU, two otherwise.  Measured (out of place, random input; one / two
+// / four waves per line): 4096 x 256 7.96 / 5.76 / 5.33 us, 4096 x 1024
+// 10.3 / 8.0 / 8.8, 4096 x 1536 13.1 / 13.7 / -, 4096 x 2048 15.0 / 16.0
+// / 16.4, 4096 x 4096 30.2 / 27.1 / 29.2, 4096 x 16384 101 / 93 / -; 2D 4 x
+// 4096 x 256 22.5 / 20.4 / -, 8 x 4096 x 256 32.3 / 31.3 / -.
 uint32_t rows_nt_nat(const tcfft_plan_s* p, const Pass& ps) {
-    if (p->rows_tiles > 0) return p->rows_tiles == 16 ? 16u : 8u;
+    if (p->rows_tiles > 0) return (uint32_t)p->rows_tiles;
     const uint32_t lines = ps.n_tiles / 16u, cus = (uint32_t)p->cus;
+    if (lines <= cus) return 4u;
     return lines > 4u * cus && lines <= 8u * cus ? 16u : 8u;
 }
 

@@ -311,8 +311,8 @@ def test_mall_chunks_bit_identical(case):
 def test_rows_tiles_per_wave_bit_identical(n, batch):
     """Tiles per wave of a contiguous-row first pass (TCFFT_OPT_ROWS_TILES 0 =
     by the grid, 4, 8, 16; 2048-point lines take at least 8; 4096-point lines
-    16, or 8 = two waves per line, k_rows_half, with one or two lines per
-    workgroup and an odd line count; a first pass with a folded twiddle, 2^18
+    16, or 8 / 4 = two / four waves per line, k_rows_half, with one or two
+    lines per workgroup and an odd line count; a first pass with a folded twiddle, 2^18
     split at 1024 points, always 16) never change results, with any
     workgroup size, and after tcfft_plan_autotune (which puts the caller's
     data back); other values are rejected."""
@@ -356,17 +356,17 @@ def test_rows_tiles_per_wave_bit_identical(n, batch):
 
 
 def test_rows_half_cached_first_pass_bit_identical():
-    """Two waves per 4096-point line as the first pass of a 2D plan (256 x
-    4096 x 3: its output stores go through the caches for the column pass
-    that follows), one or two lines per workgroup, equal one wave per line,
-    forward and inverse."""
+    """Two or four waves per 4096-point line as the first pass of a 2D plan
+    (256 x 4096 x 3: its output stores go through the caches for the column
+    pass that follows), one or two lines per workgroup, equal one wave per
+    line, forward and inverse."""
     note("[gpu] rows half, 2D 256 x 4096 x 3")
     rng = np.random.default_rng(4096)
     x = torch.from_numpy(rng.uniform(-1, 1, (3, 256, 4096, 2)).astype(np.float16)).cuda()
     plan = tcfft_amd.Plan2D(256, 4096, 3)
     for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
         outs = []
-        for tiles in (16, 8):
+        for tiles in (16, 8, 4):
             plan.set_option(tcfft_amd.OPT_ROWS_TILES, tiles)
             for wg in (1, 4):
                 plan.set_option(tcfft_amd.OPT_ROWS_WAVES, wg)
@@ -382,9 +382,9 @@ def test_rows_half_cached_first_pass_bit_identical():
 @pytest.mark.parametrize("shape", [(4096, 0, 5), (4096, 0, 1024), (8, 4096, 3)], ids=["1d-4096x5", "1d-4096x1024",
                                                                                    "2d-8x4096x3"])
 def test_rows_half_natural_bit_identical(shape):
-    """Natural-order input on 4096-point lines with two waves per line
-    (k_rows_half_nat: TCFFT_OPT_ROWS_TILES 8, the default for these grids)
-    equals one wave per line (16) and the drop-in path on host-permuted
+    """Natural-order input on 4096-point lines with two or four waves per
+    line (k_rows_half_nat: TCFFT_OPT_ROWS_TILES 8 or 4; the default picks
+    one of them) equals one wave per line (16) and the drop-in path on host-permuted
     input, bit for bit: 1D out of place and in place, 2D out of place and in
     place (through the plan's staging copy), forward and inverse."""
     a, b, batch = shape
@@ -402,7 +402,7 @@ def test_rows_half_natural_bit_identical(shape):
     for direction in (tcfft_amd.FORWARD, tcfft_amd.INVERSE):
         want = torch.from_numpy(np.ascontiguousarray(xr)).cuda()
         plan.exec(want.data_ptr(), direction)
-        for tiles in (16, 8, 0):
+        for tiles in (16, 8, 4, 0):
             plan.set_option(tcfft_amd.OPT_ROWS_TILES, tiles)
             out = torch.empty_like(src)
             plan.exec_natural(src.data_ptr(), out.data_ptr(), direction)
