@@ -270,7 +270,8 @@ int tcfft_plan_set_option(tcfft_plan plan, int option, int value);
  * of its first pass (no stagger, or 3, 5 or 7 groups with delays of 1..4
  * quanta) on `data`, keeps the fastest in the plan and reports it; then,
  * with that schedule, the workgroup size of a contiguous-row first pass
- * (TCFFT_OPT_ROWS_WAVES), its tiles per wave (TCFFT_OPT_ROWS_TILES) and the
+ * (TCFFT_OPT_ROWS_WAVES), its tiles per wave (TCFFT_OPT_ROWS_TILES; lines
+ * up to 2048 points, 4096-point lines keep the choice by the grid) and the
  * column pass delay (0..4) when the plan has a strided pass of radix
  * 512..2048.  Every timed run starts from the caller's data (a copy that
  * the call allocates: transforms repeated in place would time inf/NaN

@@ -267,7 +267,8 @@ class _Plan:
 
     def autotune(self, data_ptr: int, stream: int = 0):
         """Picks the fastest start schedule of the first pass (then its row
-        workgroup size and tiles per wave, and the column-pass delay) by
+        workgroup size, its tiles per wave for lines up to 2048 points, and
+        the column-pass delay) by
         timing forward transforms of the buffer at data_ptr, each from the
         buffer's original contents, which it holds again on return.  Returns
         (groups, delay, ms)."""
